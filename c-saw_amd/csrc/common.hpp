@@ -1,0 +1,209 @@
+// Shared host/device definitions: status plumbing, wave64 primitives and the
+// device-side views of graphs, requests and hop state.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/csaw.h"
+#include "csaw_api.h"
+#include "philox.hpp"
+
+namespace csaw {
+
+// ---------------------------------------------------------------- errors
+void set_last_error(const std::string& msg);
+const char* last_error();
+
+#define CSAW_HIP(call)                                                                      \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess) {                                                                 \
+      ::csaw::set_last_error(std::string(#call) + ": " + hipGetErrorString(e_) + " at " +   \
+                             __FILE__ + ":" + std::to_string(__LINE__));                    \
+      return e_ == hipErrorOutOfMemory ? CSAW_ERR_OOM : CSAW_ERR_HIP;                        \
+    }                                                                                       \
+  } while (0)
+
+#define CSAW_TRY(expr)               \
+  do {                               \
+    int s_ = (expr);                 \
+    if (s_ != CSAW_OK) return s_;    \
+  } while (0)
+
+#define CSAW_LAUNCH_CHECK() CSAW_HIP(hipGetLastError())
+
+// ------------------------------------------------------------ constants
+constexpr int kWave = 64;
+constexpr int kBlockEntries = 1024;  // frontier entries per count block
+constexpr int kBlockShift = 10;
+constexpr int kGroupShift = 20;      // 1024 blocks per scan group
+constexpr int kReselect = 400;       // bounded reselect (functions.cuh:844)
+constexpr int kThreadIntCap = 32;    // THREAD variant: integer CDF up to 32 entries (SML_MID)
+constexpr int kThreadFanCap = 32;    // THREAD variant: fanout up to 32
+constexpr int kWaveIntCap = 1024;    // WAVE variant: LDS integer CDF up to 1024 (MID_LRG)
+constexpr int kBlockIntCap = 8192;   // BLOCK variant: LDS integer CDF up to 8192
+constexpr int kHubMin = kWaveIntCap; // integer-bias CDFs above this live in the hub cache
+constexpr int kWaveBitmapCap = 16384;// WAVE bitmap taken-set, bits
+constexpr int kBlockBitmapCap = 65536;
+constexpr int kNumBuckets = 128;     // 0 = copy-all; 1 + bias*32 + ceil(log2 d)
+constexpr int kBucketCopy = 0;
+constexpr int kChunksPerBlock = 48;  // chunk descriptor capacity per count block
+
+// Bias kinds inside kernels: integer biases share one code path.
+enum BiasKind : int { kUniform = 0, kInt = 1, kF32 = 2 };
+
+// ----------------------------------------------------------- device views
+struct GraphDev {
+  const int64_t* off;
+  const int32_t* adj;
+  const int32_t* deg;
+  const float* w;         // weights (may be null)
+  const float* cdf;       // sequential FP32 CDF per vertex (may be null)
+  const int32_t* hub_vid;       // sorted ids of vertices with degree > kHubMin
+  const int64_t* hub_pool;      // pool offset per hub (n_hubs + 1)
+  const uint64_t* hub_cdf[2];   // [DEGREE, USER]: inclusive uint64 CDFs of hubs
+  int32_t n_hubs;
+  int64_t V, E;
+};
+
+struct ReqDev {
+  int32_t fanout[CSAW_MAX_HOPS];
+  int32_t depth, bias, dedup, samples;
+  uint32_t key_lo, key_hi, request_id, pad;
+  int64_t gbase;
+};
+
+struct Chunk {
+  uint32_t start;  // index into the order array
+  uint16_t len;
+  uint8_t bucket;
+  uint8_t hop;
+  uint32_t sum_f;
+  uint32_t max_d;
+  float sum_d;
+  uint32_t key;    // block * kNumBuckets + bucket (stable across runs)
+  uint8_t max_f;
+  uint8_t pad[3];
+};
+
+struct HopDev {
+  const int32_t* fv;    // frontier vertices
+  const int32_t* fs;    // frontier global sample ids
+  const int64_t* fo;    // per-sample frontier offsets (global sample index)
+  const int64_t* n_ptr; // number of active entries (= fo[gb])
+  int32_t hop;
+  int32_t gb;           // first inactive global sample
+  uint32_t* local_excl; // per entry
+  uint32_t* blk_excl;   // per block of 1024 entries
+  uint64_t* grp_excl;   // per group of 1024 blocks
+  int32_t* order;       // bucket-sorted entry ids (per block region)
+  Chunk* chunks;
+  uint32_t* n_chunks;
+  uint32_t chunk_cap;
+  int32_t* src;
+  int32_t* dst;
+  int32_t* smp;
+  int64_t* eo;          // per-sample edge offsets of this hop
+  uint32_t* flags;      // bit 0: chunk overflow
+};
+
+__host__ __device__ __forceinline__ int ceil_log2_u32(uint32_t d) {
+  // smallest c with 2^c >= d (d >= 1)
+#if defined(__HIP_DEVICE_COMPILE__)
+  return d <= 1 ? 0 : 32 - __clz((int)(d - 1));
+#else
+  return d <= 1 ? 0 : 32 - __builtin_clz(d - 1);
+#endif
+}
+
+// Number of edges an entry emits: depends on d only (canonical semantics).
+__host__ __device__ __forceinline__ int32_t entry_k(int64_t d, int32_t F, int32_t dedup) {
+  if (d <= 0 || F <= 0) return 0;
+  if (dedup) return d < F ? (int32_t)d : F;
+  return F;
+}
+
+__host__ __device__ __forceinline__ int bias_kind(int32_t bias) {
+  return bias == CSAW_BIAS_UNIFORM ? kUniform : (bias == CSAW_BIAS_WEIGHT ? kF32 : kInt);
+}
+
+// bucket id: 0 copy-all, else 1 + 32*kind + ceil_log2(d)  (kind 0..2, log 0..31)
+__host__ __device__ __forceinline__ int entry_bucket(int64_t d, int32_t F, int32_t dedup, int32_t bias) {
+  if (dedup && d <= F) return kBucketCopy;
+  int kind = bias_kind(bias);
+  int l = ceil_log2_u32((uint32_t)(d > 0x7fffffff ? 0x7fffffff : d));
+  return 1 + 32 * kind + l;
+}
+
+// ------------------------------------------------------- wave primitives
+#if defined(__HIP_DEVICE_COMPILE__) || defined(__HIPCC__)
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ int32_t readlane(int32_t v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ uint32_t readlane_u(uint32_t v, int l) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int l) {
+  uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+  uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Inclusive wave64 scan.
+__device__ __forceinline__ uint64_t wave_incl_scan_u64(uint64_t x) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    uint64_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  return x;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t x) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  return x;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
+
+// first index i in [0,n) with C[i] > x (C inclusive, C[n-1] > x assumed)
+template <typename T, typename Ptr>
+__device__ __forceinline__ int64_t upper_bound_dev(Ptr C, int64_t n, T x) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (C[mid] > x) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
+// hub lookup: index of v in the sorted hub id list, or -1
+__device__ __forceinline__ int32_t hub_find(const int32_t* vid, int32_t n, int32_t v) {
+  int32_t lo = 0, hi = n;
+  while (lo < hi) {
+    int32_t mid = (lo + hi) >> 1;
+    if (vid[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  return (lo < n && vid[lo] == v) ? lo : -1;
+}
+
+// Selection counter word 3.
+__device__ __forceinline__ uint32_t select_c3(uint32_t request, int hop) {
+  return (kDomSelect << 28) | (request << 8) | ((uint32_t)hop & 0xFF);
+}
+#endif
+
+}  // namespace csaw

@@ -1,0 +1,600 @@
+// Graph construction on the device: upload + validation, GPU RMAT generator,
+// degree array, FP32 sequential CDF cache and the integer hub CDF cache.
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_select.hpp>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "graph.hpp"
+
+namespace csaw {
+
+namespace {
+thread_local std::string g_last_error;
+}
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+const char* last_error() { return g_last_error.c_str(); }
+
+int device_init(int* num_cu) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+    set_last_error("no HIP device visible");
+    return CSAW_ERR_NO_DEVICE;
+  }
+  int dev = 0;
+  CSAW_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  CSAW_HIP(hipGetDeviceProperties(&prop, dev));
+  *num_cu = prop.multiProcessorCount;
+  return CSAW_OK;
+}
+
+// ------------------------------------------------------------- kernels
+__global__ void k_degree(const int64_t* __restrict__ off, int64_t V, int32_t* __restrict__ deg,
+                         unsigned long long* __restrict__ maxd, uint32_t* __restrict__ bad) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
+    int64_t d = off[v + 1] - off[v];
+    if (d < 0 || d > 0x7fffffff) {
+      atomicOr(bad, 1u);
+      d = 0;
+    }
+    deg[v] = (int32_t)d;
+    atomicMax(maxd, (unsigned long long)d);
+  }
+}
+
+__global__ void k_check_ids(const int32_t* __restrict__ adj, int64_t E, int64_t V, uint32_t* __restrict__ bad) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < E; i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t u = adj[i];
+    if (u < 0 || u >= V) atomicOr(bad, 2u);
+  }
+}
+
+__global__ void k_widen_offsets(const int32_t* __restrict__ in, int64_t n, int64_t* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = in[i];
+}
+
+// RMAT pair generation: level l of edge e uses output word (l & 3) of
+// philox(key; e lo, e hi, l >> 2, kDomRmat << 28); thresholds a, a+b,
+// a+b+c as u32 fractions; bit (scale-1-l) of (src, dst).
+__global__ void k_rmat_pairs(int scale, int64_t M, uint32_t ta, uint32_t tab, uint32_t tabc, uint32_t k0,
+                             uint32_t k1, uint64_t* __restrict__ keys) {
+  const uint64_t sentinel = (scale == 32) ? ~0ull : ((1ull << (2 * scale)) - 1);
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < M; e += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t src = 0, dst = 0;
+    U4 o{0, 0, 0, 0};
+    for (int l = 0; l < scale; ++l) {
+      if ((l & 3) == 0) o = philox4x32_10((uint32_t)e, (uint32_t)(e >> 32), (uint32_t)(l >> 2), kDomRmat << 28, k0, k1);
+      const uint32_t r = (l & 3) == 0 ? o.x : (l & 3) == 1 ? o.y : (l & 3) == 2 ? o.z : o.w;
+      const uint32_t bit = 1u << (scale - 1 - l);
+      if (r < ta) {
+      } else if (r < tab) {
+        dst |= bit;
+      } else if (r < tabc) {
+        src |= bit;
+      } else {
+        src |= bit;
+        dst |= bit;
+      }
+    }
+    uint64_t a = ((uint64_t)src << scale) | dst, b = ((uint64_t)dst << scale) | src;
+    if (src == dst) a = b = sentinel;
+    keys[2 * e] = a;
+    keys[2 * e + 1] = b;
+  }
+}
+
+// unique sorted keys -> adjacency ids + offsets (off[u] = first index of u)
+__global__ void k_keys_to_csr(const uint64_t* __restrict__ keys, int64_t E, int scale, int64_t V,
+                              int32_t* __restrict__ adj, int64_t* __restrict__ off) {
+  const uint64_t mask = (1ull << scale) - 1;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= E; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t u = i < E ? (int64_t)(keys[i] >> scale) : V;
+    const int64_t p = i > 0 ? (int64_t)(keys[i - 1] >> scale) : -1;
+    if (i < E) adj[i] = (int32_t)(keys[i] & mask);
+    for (int64_t v = p + 1; v <= u; ++v) off[v] = i;
+  }
+}
+
+__global__ void k_weights(uint32_t k0, uint32_t k1, int64_t E, float* __restrict__ w) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < E; e += (int64_t)gridDim.x * blockDim.x) {
+    U4 o = philox4x32_10((uint32_t)e, (uint32_t)(e >> 32), 0u, kDomWeights << 28, k0, k1);
+    w[e] = (float)((o.x >> 8) + 1u) * 5.9604644775390625e-08f;
+  }
+}
+
+// Sequential FP32 CDF per vertex, strictly left to right (one thread per
+// vertex; never a parallel scan, whose rounding would differ).
+__global__ void k_fp32_cdf(const int64_t* __restrict__ off, const float* __restrict__ w, int64_t V,
+                           float* __restrict__ cdf) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
+    float acc = 0.0f;
+    const int64_t b = off[v], e = off[v + 1];
+    for (int64_t i = b; i < e; ++i) {
+      acc = __fadd_rn(acc, w[i]);
+      cdf[i] = acc;
+    }
+  }
+}
+
+__global__ void k_seeds(uint32_t k0, uint32_t k1, uint32_t request, int64_t V, int64_t n, int32_t* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    U4 o = philox4x32_10((uint32_t)i, (uint32_t)(i >> 32), request, kDomSeeds << 28, k0, k1);
+    out[i] = (int32_t)mulhi64(philox_t64(o), (uint64_t)V);
+  }
+}
+
+// Hub cache, split across workgroups: each workgroup scans one segment of
+// one hub's neighbour list (kHubSeg entries) and writes the segment-local
+// inclusive CDF plus the segment total; a second pass adds segment offsets.
+constexpr int kHubSeg = 4096;
+constexpr int kHubThreads = 256;
+
+template <int KIND>  // 0 degree, 1 user hook
+__global__ __launch_bounds__(kHubThreads) void k_hub_segments(GraphDev G, csaw_graph_view hv, const int32_t* __restrict__ seg_hub,
+                                                              const int32_t* __restrict__ seg_idx, int64_t n_seg,
+                                                              uint64_t* __restrict__ pool, uint64_t* __restrict__ seg_tot) {
+  __shared__ uint64_t wsum[kHubThreads / kWave];
+  __shared__ uint64_t carry_s;
+  for (int64_t s = blockIdx.x; s < n_seg; s += gridDim.x) {
+    const int32_t h = seg_hub[s];
+    const int32_t v = G.hub_vid[h];
+    const int64_t o = G.off[v];
+    const int64_t d = G.off[v + 1] - o;
+    const int64_t b = (int64_t)seg_idx[s] * kHubSeg;
+    const int64_t e = b + kHubSeg < d ? b + kHubSeg : d;
+    const int64_t pbase = G.hub_pool[h];
+    if (threadIdx.x == 0) carry_s = 0;
+    __syncthreads();
+    for (int64_t i0 = b; i0 < e; i0 += kHubThreads) {
+      const int64_t i = i0 + threadIdx.x;
+      uint64_t wv = 0;
+      if (i < e) {
+        const int32_t u = G.adj[o + i];
+        wv = KIND == 0 ? (uint64_t)G.deg[u] : (uint64_t)EdgeBias(u, &hv);
+      }
+      uint64_t sc = wave_incl_scan_u64(wv);
+      const int wid = threadIdx.x / kWave;
+      if (lane_id() == kWave - 1) wsum[wid] = sc;
+      __syncthreads();
+      uint64_t pre = carry_s;
+      for (int k = 0; k < wid; ++k) pre += wsum[k];
+      if (i < e) pool[pbase + i] = pre + sc;
+      __syncthreads();
+      if (threadIdx.x == kHubThreads - 1) carry_s = pre + sc;
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) seg_tot[s] = carry_s;
+    __syncthreads();
+  }
+}
+
+// seg_first[h]..seg_first[h+1]: segments of hub h; exclusive offsets per segment
+__global__ void k_hub_seg_offsets(const int64_t* __restrict__ seg_first, int32_t n_hubs, uint64_t* __restrict__ seg_tot) {
+  for (int32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < n_hubs; h += gridDim.x * blockDim.x) {
+    uint64_t acc = 0;
+    for (int64_t s = seg_first[h]; s < seg_first[h + 1]; ++s) {
+      uint64_t t = seg_tot[s];
+      seg_tot[s] = acc;
+      acc += t;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kHubThreads) void k_hub_add(GraphDev G, const int32_t* __restrict__ seg_hub,
+                                                        const int32_t* __restrict__ seg_idx, int64_t n_seg,
+                                                        const uint64_t* __restrict__ seg_off, uint64_t* __restrict__ pool) {
+  for (int64_t s = blockIdx.x; s < n_seg; s += gridDim.x) {
+    const int32_t h = seg_hub[s];
+    const int32_t v = G.hub_vid[h];
+    const int64_t d = G.off[v + 1] - G.off[v];
+    const int64_t b = (int64_t)seg_idx[s] * kHubSeg;
+    const int64_t e = b + kHubSeg < d ? b + kHubSeg : d;
+    const uint64_t add = seg_off[s];
+    if (add == 0) continue;
+    for (int64_t i = b + threadIdx.x; i < e; i += kHubThreads) pool[G.hub_pool[h] + i] += add;
+  }
+}
+
+static int grid_for(int64_t n, int threads, int num_cu) {
+  int64_t g = (n + threads - 1) / threads;
+  int64_t cap = (int64_t)num_cu * 32;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+int graph_alloc_common(csaw_graph* g) {
+  CSAW_TRY(device_init(&g->num_cu));
+  CSAW_HIP(hipGetDevice(&g->device));
+  CSAW_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+  return CSAW_OK;
+}
+
+// Degree array + validation.
+static int finish_graph(csaw_graph* g) {
+  CSAW_HIP(hipMalloc(&g->deg, sizeof(int32_t) * (g->V > 0 ? g->V : 1)));
+  unsigned long long* dmax;
+  uint32_t* bad;
+  CSAW_HIP(hipMalloc(&dmax, sizeof(unsigned long long) + sizeof(uint32_t)));
+  bad = (uint32_t*)(dmax + 1);
+  CSAW_HIP(hipMemsetAsync(dmax, 0, sizeof(unsigned long long) + sizeof(uint32_t), g->stream));
+  if (g->V > 0) {
+    k_degree<<<grid_for(g->V, 256, g->num_cu), 256, 0, g->stream>>>(g->off, g->V, g->deg, dmax, bad);
+    CSAW_LAUNCH_CHECK();
+  }
+  if (g->E > 0) {
+    k_check_ids<<<grid_for(g->E, 256, g->num_cu), 256, 0, g->stream>>>(g->adj, g->E, g->V, bad);
+    CSAW_LAUNCH_CHECK();
+  }
+  unsigned long long hm = 0;
+  uint32_t hb = 0;
+  CSAW_HIP(hipMemcpyAsync(&hm, dmax, sizeof(hm), hipMemcpyDeviceToHost, g->stream));
+  CSAW_HIP(hipMemcpyAsync(&hb, bad, sizeof(hb), hipMemcpyDeviceToHost, g->stream));
+  CSAW_HIP(hipStreamSynchronize(g->stream));
+  CSAW_HIP(hipFree(dmax));
+  g->max_deg = (int64_t)hm;
+  if (hb) {
+    set_last_error(hb & 1 ? "offsets not monotone or degree >= 2^31" : "neighbour id out of range");
+    return CSAW_ERR_INVALID_GRAPH;
+  }
+  return CSAW_OK;
+}
+
+int graph_prepare_bias(csaw_graph* g, int32_t bias) {
+  if (bias == CSAW_BIAS_WEIGHT) {
+    if (!g->w) {
+      set_last_error("CSAW_BIAS_WEIGHT needs edge weights");
+      return CSAW_ERR_INVALID_ARG;
+    }
+    if (!g->cdf) {
+      CSAW_HIP(hipMalloc(&g->cdf, sizeof(float) * (g->E > 0 ? g->E : 1)));
+      if (g->V > 0) {
+        k_fp32_cdf<<<grid_for(g->V, 256, g->num_cu), 256, 0, g->stream>>>(g->off, g->w, g->V, g->cdf);
+        CSAW_LAUNCH_CHECK();
+      }
+      CSAW_HIP(hipStreamSynchronize(g->stream));
+    }
+    return CSAW_OK;
+  }
+  if (bias != CSAW_BIAS_DEGREE && bias != CSAW_BIAS_USER) return CSAW_OK;
+  const int kind = bias == CSAW_BIAS_DEGREE ? 0 : 1;
+  if (g->n_hubs < 0) {
+    std::vector<int32_t> deg(g->V);
+    if (g->V > 0) CSAW_HIP(hipMemcpy(deg.data(), g->deg, sizeof(int32_t) * g->V, hipMemcpyDeviceToHost));
+    std::vector<int32_t> vid;
+    std::vector<int64_t> pool{0};
+    for (int64_t v = 0; v < g->V; ++v)
+      if (deg[v] > kHubMin) {
+        vid.push_back((int32_t)v);
+        pool.push_back(pool.back() + deg[v]);
+      }
+    g->n_hubs = (int32_t)vid.size();
+    g->hub_entries = pool.back();
+    CSAW_HIP(hipMalloc(&g->hub_vid, sizeof(int32_t) * (vid.size() + 1)));
+    CSAW_HIP(hipMalloc(&g->hub_pool, sizeof(int64_t) * pool.size()));
+    if (!vid.empty())
+      CSAW_HIP(hipMemcpy(g->hub_vid, vid.data(), sizeof(int32_t) * vid.size(), hipMemcpyHostToDevice));
+    CSAW_HIP(hipMemcpy(g->hub_pool, pool.data(), sizeof(int64_t) * pool.size(), hipMemcpyHostToDevice));
+  }
+  if (g->hub_cdf[kind] || g->n_hubs == 0) return CSAW_OK;
+  // segment work list
+  std::vector<int64_t> pool(g->n_hubs + 1);
+  CSAW_HIP(hipMemcpy(pool.data(), g->hub_pool, sizeof(int64_t) * pool.size(), hipMemcpyDeviceToHost));
+  std::vector<int32_t> seg_hub, seg_idx;
+  std::vector<int64_t> seg_first{0};
+  for (int32_t h = 0; h < g->n_hubs; ++h) {
+    int64_t d = pool[h + 1] - pool[h];
+    int64_t ns = (d + kHubSeg - 1) / kHubSeg;
+    for (int64_t s = 0; s < ns; ++s) {
+      seg_hub.push_back(h);
+      seg_idx.push_back((int32_t)s);
+    }
+    seg_first.push_back((int64_t)seg_hub.size());
+  }
+  const int64_t n_seg = (int64_t)seg_hub.size();
+  int32_t *d_seg_hub, *d_seg_idx;
+  int64_t* d_seg_first;
+  uint64_t* d_seg_tot;
+  CSAW_HIP(hipMalloc(&g->hub_cdf[kind], sizeof(uint64_t) * g->hub_entries));
+  CSAW_HIP(hipMalloc(&d_seg_hub, sizeof(int32_t) * n_seg));
+  CSAW_HIP(hipMalloc(&d_seg_idx, sizeof(int32_t) * n_seg));
+  CSAW_HIP(hipMalloc(&d_seg_first, sizeof(int64_t) * seg_first.size()));
+  CSAW_HIP(hipMalloc(&d_seg_tot, sizeof(uint64_t) * n_seg));
+  CSAW_HIP(hipMemcpy(d_seg_hub, seg_hub.data(), sizeof(int32_t) * n_seg, hipMemcpyHostToDevice));
+  CSAW_HIP(hipMemcpy(d_seg_idx, seg_idx.data(), sizeof(int32_t) * n_seg, hipMemcpyHostToDevice));
+  CSAW_HIP(hipMemcpy(d_seg_first, seg_first.data(), sizeof(int64_t) * seg_first.size(), hipMemcpyHostToDevice));
+  GraphDev G = g->view();
+  csaw_graph_view hv = g->hook_view();
+  const int grid = (int)std::min<int64_t>(n_seg, (int64_t)g->num_cu * 8);
+  if (kind == 0)
+    k_hub_segments<0><<<grid, kHubThreads, 0, g->stream>>>(G, hv, d_seg_hub, d_seg_idx, n_seg, g->hub_cdf[kind], d_seg_tot);
+  else
+    k_hub_segments<1><<<grid, kHubThreads, 0, g->stream>>>(G, hv, d_seg_hub, d_seg_idx, n_seg, g->hub_cdf[kind], d_seg_tot);
+  CSAW_LAUNCH_CHECK();
+  k_hub_seg_offsets<<<grid_for(g->n_hubs, 64, g->num_cu), 64, 0, g->stream>>>(d_seg_first, g->n_hubs, d_seg_tot);
+  CSAW_LAUNCH_CHECK();
+  k_hub_add<<<grid, kHubThreads, 0, g->stream>>>(G, d_seg_hub, d_seg_idx, n_seg, d_seg_tot, g->hub_cdf[kind]);
+  CSAW_LAUNCH_CHECK();
+  CSAW_HIP(hipStreamSynchronize(g->stream));
+  CSAW_HIP(hipFree(d_seg_hub));
+  CSAW_HIP(hipFree(d_seg_idx));
+  CSAW_HIP(hipFree(d_seg_first));
+  CSAW_HIP(hipFree(d_seg_tot));
+  return CSAW_OK;
+}
+
+}  // namespace csaw
+
+using namespace csaw;
+
+csaw::GraphDev csaw_graph::view() const {
+  GraphDev G;
+  G.off = off;
+  G.adj = adj;
+  G.deg = deg;
+  G.w = w;
+  G.cdf = cdf;
+  G.hub_vid = hub_vid;
+  G.hub_pool = hub_pool;
+  G.hub_cdf[0] = hub_cdf[0];
+  G.hub_cdf[1] = hub_cdf[1];
+  G.n_hubs = n_hubs < 0 ? 0 : n_hubs;
+  G.V = V;
+  G.E = E;
+  return G;
+}
+
+csaw_graph_view csaw_graph::hook_view() const {
+  csaw_graph_view h;
+  h.beg_pos = off;
+  h.adj_list = adj;
+  h.degree_list = deg;
+  h.weight_list = w;
+  h.vert_count = V;
+  h.edge_count = E;
+  return h;
+}
+
+static void graph_free(csaw_graph* g) {
+  if (!g) return;
+  (void)hipFree(g->off);
+  (void)hipFree(g->adj);
+  (void)hipFree(g->deg);
+  (void)hipFree(g->w);
+  (void)hipFree(g->cdf);
+  (void)hipFree(g->hub_vid);
+  (void)hipFree(g->hub_pool);
+  (void)hipFree(g->hub_cdf[0]);
+  (void)hipFree(g->hub_cdf[1]);
+  if (g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
+}
+
+extern "C" int csaw_graph_destroy(csaw_graph* g) {
+  graph_free(g);
+  return CSAW_OK;
+}
+
+extern "C" int csaw_graph_create(const void* offsets, int offsets_are_i64, const int32_t* indices,
+                                 const float* weights, int64_t V, int64_t E, csaw_graph** out) {
+  if (!out || !offsets || (E > 0 && !indices) || V < 0 || E < 0 || V > 0x7fffffffLL) {
+    set_last_error("csaw_graph_create: invalid argument");
+    return CSAW_ERR_INVALID_ARG;
+  }
+  *out = nullptr;
+  // host-side checks of the offsets (the reference asserts nothing, graph.hpp:37)
+  for (int64_t v = 0; v <= V; ++v) {
+    int64_t x = offsets_are_i64 ? ((const int64_t*)offsets)[v] : ((const int32_t*)offsets)[v];
+    int64_t px = v ? (offsets_are_i64 ? ((const int64_t*)offsets)[v - 1] : ((const int32_t*)offsets)[v - 1]) : 0;
+    if ((v == 0 && x != 0) || x < px || (v == V && x != E)) {
+      set_last_error("csaw_graph_create: offsets must start at 0, be monotone and end at E");
+      return CSAW_ERR_INVALID_GRAPH;
+    }
+  }
+  csaw_graph* g = new csaw_graph();
+  int st = graph_alloc_common(g);
+  if (st != CSAW_OK) { graph_free(g); return st; }
+  g->V = V;
+  g->E = E;
+  auto body = [&]() -> int {
+    CSAW_HIP(hipMalloc(&g->off, sizeof(int64_t) * (V + 1)));
+    CSAW_HIP(hipMalloc(&g->adj, sizeof(int32_t) * (E > 0 ? E : 1)));
+    if (offsets_are_i64) {
+      CSAW_HIP(hipMemcpy(g->off, offsets, sizeof(int64_t) * (V + 1), hipMemcpyHostToDevice));
+    } else {
+      int32_t* tmp;
+      CSAW_HIP(hipMalloc(&tmp, sizeof(int32_t) * (V + 1)));
+      CSAW_HIP(hipMemcpy(tmp, offsets, sizeof(int32_t) * (V + 1), hipMemcpyHostToDevice));
+      k_widen_offsets<<<grid_for(V + 1, 256, g->num_cu), 256, 0, g->stream>>>(tmp, V + 1, g->off);
+      CSAW_LAUNCH_CHECK();
+      CSAW_HIP(hipStreamSynchronize(g->stream));
+      CSAW_HIP(hipFree(tmp));
+    }
+    if (E > 0) CSAW_HIP(hipMemcpy(g->adj, indices, sizeof(int32_t) * E, hipMemcpyHostToDevice));
+    if (weights) {
+      CSAW_HIP(hipMalloc(&g->w, sizeof(float) * (E > 0 ? E : 1)));
+      if (E > 0) CSAW_HIP(hipMemcpy(g->w, weights, sizeof(float) * E, hipMemcpyHostToDevice));
+    }
+    return finish_graph(g);
+  };
+  st = body();
+  if (st != CSAW_OK) { graph_free(g); return st; }
+  *out = g;
+  return CSAW_OK;
+}
+
+static uint32_t thr32(double x) {
+  if (x <= 0.0) return 0;
+  if (x >= 1.0) return 0xFFFFFFFFu;
+  return (uint32_t)(x * 4294967296.0);
+}
+
+extern "C" int csaw_graph_create_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t key,
+                                      int with_weights, csaw_graph** out) {
+  if (!out || scale < 1 || scale > 30 || edge_factor < 1 || a < 0 || b < 0 || c < 0 || a + b + c > 1.0) {
+    set_last_error("csaw_graph_create_rmat: invalid argument");
+    return CSAW_ERR_INVALID_ARG;
+  }
+  *out = nullptr;
+  csaw_graph* g = new csaw_graph();
+  int st = graph_alloc_common(g);
+  if (st != CSAW_OK) { graph_free(g); return st; }
+  auto body = [&]() -> int {
+    const int64_t V = (int64_t)1 << scale;
+    const int64_t M = (int64_t)edge_factor << scale;
+    const size_t n = (size_t)(2 * M);
+    uint64_t *k0, *k1;
+    CSAW_HIP(hipMalloc(&k0, sizeof(uint64_t) * n));
+    CSAW_HIP(hipMalloc(&k1, sizeof(uint64_t) * n));
+    k_rmat_pairs<<<grid_for(M, 256, g->num_cu), 256, 0, g->stream>>>(scale, M, thr32(a), thr32(a + b), thr32(a + b + c),
+                                                                     (uint32_t)key, (uint32_t)(key >> 32), k0);
+    CSAW_LAUNCH_CHECK();
+    rocprim::double_buffer<uint64_t> db(k0, k1);
+    size_t tmp_bytes = 0;
+    CSAW_HIP(rocprim::radix_sort_keys(nullptr, tmp_bytes, db, n, 0, 2 * scale, g->stream));
+    void* tmp;
+    CSAW_HIP(hipMalloc(&tmp, tmp_bytes));
+    CSAW_HIP(rocprim::radix_sort_keys(tmp, tmp_bytes, db, n, 0, 2 * scale, g->stream));
+    CSAW_HIP(hipFree(tmp));
+    uint64_t* sorted = db.current();
+    uint64_t* uniq = db.alternate();
+    uint32_t* d_cnt;
+    CSAW_HIP(hipMalloc(&d_cnt, sizeof(uint32_t)));
+    tmp_bytes = 0;
+    CSAW_HIP(rocprim::unique(nullptr, tmp_bytes, sorted, uniq, d_cnt, n, rocprim::equal_to<uint64_t>(), g->stream));
+    CSAW_HIP(hipMalloc(&tmp, tmp_bytes));
+    CSAW_HIP(rocprim::unique(tmp, tmp_bytes, sorted, uniq, d_cnt, n, rocprim::equal_to<uint64_t>(), g->stream));
+    uint32_t cnt = 0;
+    CSAW_HIP(hipMemcpyAsync(&cnt, d_cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
+    CSAW_HIP(hipStreamSynchronize(g->stream));
+    CSAW_HIP(hipFree(tmp));
+    CSAW_HIP(hipFree(d_cnt));
+    int64_t E = cnt;
+    if (E > 0) {
+      uint64_t last = 0;
+      CSAW_HIP(hipMemcpy(&last, uniq + E - 1, sizeof(uint64_t), hipMemcpyDeviceToHost));
+      if (last == ((1ull << (2 * scale)) - 1)) --E;  // self-loop sentinel
+    }
+    g->V = V;
+    g->E = E;
+    CSAW_HIP(hipMalloc(&g->off, sizeof(int64_t) * (V + 1)));
+    CSAW_HIP(hipMalloc(&g->adj, sizeof(int32_t) * (E > 0 ? E : 1)));
+    k_keys_to_csr<<<grid_for(E + 1, 256, g->num_cu), 256, 0, g->stream>>>(uniq, E, scale, V, g->adj, g->off);
+    CSAW_LAUNCH_CHECK();
+    CSAW_HIP(hipStreamSynchronize(g->stream));
+    CSAW_HIP(hipFree(k0));
+    CSAW_HIP(hipFree(k1));
+    if (with_weights) {
+      CSAW_HIP(hipMalloc(&g->w, sizeof(float) * (E > 0 ? E : 1)));
+      if (E > 0) {
+        k_weights<<<grid_for(E, 256, g->num_cu), 256, 0, g->stream>>>((uint32_t)key, (uint32_t)(key >> 32), E, g->w);
+        CSAW_LAUNCH_CHECK();
+      }
+    }
+    return finish_graph(g);
+  };
+  st = body();
+  if (st != CSAW_OK) { graph_free(g); return st; }
+  *out = g;
+  return CSAW_OK;
+}
+
+// The reference's on-disk CSR (graph.hpp:18-74): beg.bin holds V+1 int64
+// offsets, csr.bin E int64 neighbour ids; both are narrowed on load (ids to
+// int32).  The optional weight file holds E float32 values.
+static int read_file(const char* path, std::vector<char>& buf) {
+  FILE* f = fopen(path, "rb");
+  if (!f) {
+    set_last_error(std::string("cannot open ") + path);
+    return CSAW_ERR_IO;
+  }
+  fseek(f, 0, SEEK_END);
+  long sz = ftell(f);
+  fseek(f, 0, SEEK_SET);
+  buf.resize(sz > 0 ? (size_t)sz : 0);
+  size_t got = sz > 0 ? fread(buf.data(), 1, (size_t)sz, f) : 0;
+  fclose(f);
+  if ((long)got != sz) {
+    set_last_error(std::string("short read of ") + path);
+    return CSAW_ERR_IO;
+  }
+  return CSAW_OK;
+}
+
+extern "C" int csaw_graph_load_bin(const char* beg_file, const char* csr_file, const char* weight_file,
+                                   csaw_graph** out) {
+  if (!beg_file || !csr_file || !out) return CSAW_ERR_INVALID_ARG;
+  std::vector<char> beg, csr, wf;
+  CSAW_TRY(read_file(beg_file, beg));
+  CSAW_TRY(read_file(csr_file, csr));
+  if (beg.size() < 8 || beg.size() % 8 || csr.size() % 8) {
+    set_last_error("beg.bin / csr.bin sizes are not multiples of 8 bytes");
+    return CSAW_ERR_INVALID_GRAPH;
+  }
+  const int64_t V = (int64_t)(beg.size() / 8) - 1;
+  const int64_t* off = (const int64_t*)beg.data();
+  const int64_t E = off[V];
+  if (E < 0 || (int64_t)(csr.size() / 8) < E) {
+    set_last_error("csr.bin holds fewer ids than beg.bin's last offset");
+    return CSAW_ERR_INVALID_GRAPH;
+  }
+  std::vector<int32_t> adj(E);
+  const int64_t* ids = (const int64_t*)csr.data();
+  for (int64_t i = 0; i < E; ++i) {
+    if (ids[i] < 0 || ids[i] >= V) {
+      set_last_error("csr.bin id out of range");
+      return CSAW_ERR_INVALID_GRAPH;
+    }
+    adj[i] = (int32_t)ids[i];
+  }
+  const float* w = nullptr;
+  if (weight_file) {
+    CSAW_TRY(read_file(weight_file, wf));
+    if ((int64_t)(wf.size() / 4) < E) {
+      set_last_error("weight file holds fewer than E float32 values");
+      return CSAW_ERR_INVALID_GRAPH;
+    }
+    w = (const float*)wf.data();
+  }
+  return csaw_graph_create(off, 1, adj.data(), w, V, E, out);
+}
+
+extern "C" int csaw_graph_info(const csaw_graph* g, int64_t* V, int64_t* E, int64_t* max_degree) {
+  if (!g) return CSAW_ERR_INVALID_ARG;
+  if (V) *V = g->V;
+  if (E) *E = g->E;
+  if (max_degree) *max_degree = g->max_deg;
+  return CSAW_OK;
+}
+
+extern "C" int csaw_graph_copy_csr(const csaw_graph* g, int64_t* offsets, int32_t* indices, float* weights) {
+  if (!g) return CSAW_ERR_INVALID_ARG;
+  if (offsets) CSAW_HIP(hipMemcpy(offsets, g->off, sizeof(int64_t) * (g->V + 1), hipMemcpyDeviceToHost));
+  if (indices && g->E > 0) CSAW_HIP(hipMemcpy(indices, g->adj, sizeof(int32_t) * g->E, hipMemcpyDeviceToHost));
+  if (weights) {
+    if (!g->w) return CSAW_ERR_INVALID_ARG;
+    if (g->E > 0) CSAW_HIP(hipMemcpy(weights, g->w, sizeof(float) * g->E, hipMemcpyDeviceToHost));
+  }
+  return CSAW_OK;
+}
+
+extern "C" int csaw_seeds(const csaw_graph* g, uint64_t key, uint32_t request, int64_t n, int32_t* out) {
+  if (!g || n < 0 || (n > 0 && !out) || g->V < 1) return CSAW_ERR_INVALID_ARG;
+  if (n == 0) return CSAW_OK;
+  int32_t* d;
+  CSAW_HIP(hipMalloc(&d, sizeof(int32_t) * n));
+  k_seeds<<<grid_for(n, 256, g->num_cu), 256, 0, g->stream>>>((uint32_t)key, (uint32_t)(key >> 32), request, g->V, n, d);
+  CSAW_LAUNCH_CHECK();
+  CSAW_HIP(hipMemcpyAsync(out, d, sizeof(int32_t) * n, hipMemcpyDeviceToHost, g->stream));
+  CSAW_HIP(hipStreamSynchronize(g->stream));
+  CSAW_HIP(hipFree(d));
+  return CSAW_OK;
+}
+
+extern "C" const char* csaw_last_error(void) { return csaw::last_error(); }

@@ -1,0 +1,35 @@
+// Device-resident CSR graph (replaces gpu_graph, non-stream/gpu_graph.cuh:10-65,
+// and the host loader graph<>, non-stream/graph.hpp:4-111).
+#pragma once
+#include <vector>
+
+#include "common.hpp"
+
+struct csaw_graph {
+  int64_t V = 0, E = 0, max_deg = 0;
+  int64_t* off = nullptr;  // V+1
+  int32_t* adj = nullptr;  // E
+  int32_t* deg = nullptr;  // V
+  float* w = nullptr;      // E (optional)
+  float* cdf = nullptr;    // E (optional, built on demand)
+  // hub cache: vertices with degree > kHubMin, sorted by id
+  int32_t n_hubs = -1;     // -1: not computed
+  int32_t* hub_vid = nullptr;
+  int64_t* hub_pool = nullptr;  // n_hubs+1 pool offsets
+  uint64_t* hub_cdf[2] = {nullptr, nullptr};  // [DEGREE, USER]
+  int64_t hub_entries = 0;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int num_cu = 0;
+
+  csaw::GraphDev view() const;
+  csaw_graph_view hook_view() const;
+};
+
+namespace csaw {
+// Builds lazily what a bias kind needs: FP32 CDF (CSAW_BIAS_WEIGHT) or the
+// integer hub cache (CSAW_BIAS_DEGREE / CSAW_BIAS_USER).
+int graph_prepare_bias(csaw_graph* g, int32_t bias);
+int graph_alloc_common(csaw_graph* g);
+int device_init(int* num_cu);
+}  // namespace csaw

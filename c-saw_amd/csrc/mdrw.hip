@@ -1,0 +1,153 @@
+// Multi-dimensional random walk (FrontierSize > 1): the reference's
+// check_layer (non-stream/main.cu:95-194) with ITS_MDRW (functions.cuh:1014-1060).
+// Per sample and step: pick frontier slot q with probability proportional to
+// VertexBias(frontier[q]) (draw philox(key; t, 0, sample, DOM_MDRW|request)),
+// then a uniform neighbour nb of u = frontier[q] (draw (t, 1, ...)); a degree-0
+// pick consumes the step (main.cu:162-163); emit (u, nb); frontier[q] = nb.
+//
+// One wave per sample; frontier and its VertexBias values live in LDS, the
+// total T is kept incrementally and the slot search streams a wave scan over
+// the bias values (no stored CDF).  Unlike the reference (D11: only thread 0
+// of the grid commits), every sample is walked.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "graph.hpp"
+#include "select.hpp"
+
+namespace csaw {
+
+constexpr int kMdrwMaxFs = 4096;
+
+__global__ __launch_bounds__(64) void k_mdrw(GraphDev G, csaw_graph_view hv, uint32_t k0, uint32_t k1, uint32_t request,
+                                            int32_t S, int32_t fs, int32_t depth, const int32_t* __restrict__ seeds,
+                                            int32_t* __restrict__ src, int32_t* __restrict__ dst,
+                                            int32_t* __restrict__ cnt) {
+  __shared__ int32_t front[kMdrwMaxFs];
+  __shared__ uint32_t wts[kMdrwMaxFs];
+  const int lane = lane_id();
+  const uint32_t c3 = (kDomMdrw << 28) | request;
+  for (int32_t s = blockIdx.x; s < S; s += gridDim.x) {
+    uint64_t T = 0;
+    for (int q0 = 0; q0 < fs; q0 += kWave) {
+      const int q = q0 + lane;
+      uint32_t w = 0;
+      if (q < fs) {
+        const int32_t v = seeds[(int64_t)s * fs + q];
+        front[q] = v;
+        w = VertexBias(v, &hv);
+        wts[q] = w;
+      }
+      T += readlane_u64(wave_incl_scan_u64((uint64_t)w), kWave - 1);
+    }
+    wave_fence();
+    int32_t c = 0;
+    for (int32_t t = 0; t < depth; ++t) {
+      const U4 o = philox4x32_10((uint32_t)t, 0u, (uint32_t)s, c3, k0, k1);
+      int32_t q = 0;
+      if (T == 0) {
+        q = (int32_t)mulhi64(philox_t64(o), (uint64_t)fs);
+      } else {
+        const uint64_t x = mulhi64(philox_t64(o), T);
+        uint64_t acc = 0;
+        for (int q0 = 0; q0 < fs; q0 += kWave) {
+          const int qi = q0 + lane;
+          const uint64_t w = qi < fs ? wts[qi] : 0u;
+          const uint64_t sc = wave_incl_scan_u64(w);
+          const uint64_t tot = readlane_u64(sc, kWave - 1);
+          if (acc + tot > x) {
+            q = q0 + __builtin_ctzll(ballot(acc + sc > x));
+            break;
+          }
+          acc += tot;
+        }
+      }
+      const int32_t u = front[q];
+      const int32_t d = G.deg[u];
+      if (d == 0) continue;
+      const U4 o2 = philox4x32_10((uint32_t)t, 1u, (uint32_t)s, c3, k0, k1);
+      const int32_t nb = G.adj[G.off[u] + (int64_t)mulhi64(philox_t64(o2), (uint64_t)d)];
+      if (lane == 0) {
+        src[(int64_t)s * depth + c] = u;
+        dst[(int64_t)s * depth + c] = nb;
+        front[q] = nb;
+        wts[q] = VertexBias(nb, &hv);
+      }
+      T = T - (uint64_t)VertexBias(u, &hv) + (uint64_t)VertexBias(nb, &hv);
+      ++c;
+      wave_fence();
+    }
+    if (lane == 0) cnt[s] = c;
+  }
+}
+
+}  // namespace csaw
+
+using namespace csaw;
+
+extern "C" int csaw_mdrw(csaw_graph* g, uint64_t seed, uint32_t request_id, int32_t samples, int32_t frontier_size,
+                         int32_t depth, const int32_t* seeds, int32_t* src, int32_t* dst, int64_t* offsets,
+                         csaw_stats* stats) {
+  if (!g || samples < 0 || frontier_size < 1 || frontier_size > kMdrwMaxFs || depth < 0 || !offsets ||
+      (samples > 0 && (!seeds || (depth > 0 && (!src || !dst)))))
+    return CSAW_ERR_INVALID_ARG;
+  const int64_t ns = (int64_t)samples * frontier_size;
+  for (int64_t i = 0; i < ns; ++i)
+    if (seeds[i] < 0 || seeds[i] >= g->V) return CSAW_ERR_INVALID_ARG;
+  const int64_t slots = std::max<int64_t>((int64_t)samples * depth, 1);
+  int32_t *d_seeds = nullptr, *d_src = nullptr, *d_dst = nullptr, *d_cnt = nullptr;
+  auto cleanup = [&]() {
+    (void)hipFree(d_seeds);
+    (void)hipFree(d_src);
+    (void)hipFree(d_dst);
+    (void)hipFree(d_cnt);
+  };
+  auto body = [&]() -> int {
+    CSAW_HIP(hipMalloc(&d_seeds, sizeof(int32_t) * std::max<int64_t>(ns, 1)));
+    CSAW_HIP(hipMalloc(&d_src, sizeof(int32_t) * slots));
+    CSAW_HIP(hipMalloc(&d_dst, sizeof(int32_t) * slots));
+    CSAW_HIP(hipMalloc(&d_cnt, sizeof(int32_t) * std::max<int32_t>(samples, 1)));
+    if (ns > 0) CSAW_HIP(hipMemcpy(d_seeds, seeds, sizeof(int32_t) * ns, hipMemcpyHostToDevice));
+    hipEvent_t e0, e1;
+    CSAW_HIP(hipEventCreate(&e0));
+    CSAW_HIP(hipEventCreate(&e1));
+    CSAW_HIP(hipEventRecord(e0, g->stream));
+    if (samples > 0) {
+      const int grid = std::min<int>(samples, g->num_cu * 16);
+      k_mdrw<<<grid, 64, 0, g->stream>>>(g->view(), g->hook_view(), (uint32_t)seed, (uint32_t)(seed >> 32), request_id,
+                                         samples, frontier_size, depth, d_seeds, d_src, d_dst, d_cnt);
+      CSAW_LAUNCH_CHECK();
+    }
+    CSAW_HIP(hipEventRecord(e1, g->stream));
+    CSAW_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    CSAW_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::vector<int32_t> cnt(samples), hs(slots), hd(slots);
+    if (samples > 0) {
+      CSAW_HIP(hipMemcpy(cnt.data(), d_cnt, sizeof(int32_t) * samples, hipMemcpyDeviceToHost));
+      CSAW_HIP(hipMemcpy(hs.data(), d_src, sizeof(int32_t) * slots, hipMemcpyDeviceToHost));
+      CSAW_HIP(hipMemcpy(hd.data(), d_dst, sizeof(int32_t) * slots, hipMemcpyDeviceToHost));
+    }
+    int64_t at = 0;
+    for (int32_t s = 0; s < samples; ++s) {
+      offsets[s] = at;
+      for (int32_t i = 0; i < cnt[s]; ++i, ++at) {
+        src[at] = hs[(int64_t)s * depth + i];
+        dst[at] = hd[(int64_t)s * depth + i];
+      }
+    }
+    offsets[samples] = at;
+    if (stats) {
+      memset(stats, 0, sizeof(*stats));
+      stats->edges = at;
+      stats->kernel_ms = ms;
+    }
+    return CSAW_OK;
+  };
+  int st = body();
+  cleanup();
+  return st;
+}

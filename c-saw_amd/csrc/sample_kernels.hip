@@ -1,0 +1,542 @@
+// Per-hop sampling pipeline (all sizes device-resident, fixed persistent grids,
+// so a whole multi-hop run can be captured in one hipGraph):
+//
+//   k_count<BIN>    entry -> k (edges it emits), block-local exclusive offsets,
+//                   block sums; with BIN also a per-block counting sort of the
+//                   entries by (bias kind, ceil log2 degree) into chunks.
+//   k_scan_blocks   exclusive scan of block sums within groups of 1024 blocks
+//   k_scan_groups   exclusive scan of group sums; hop total
+//   k_offsets       per-sample edge offsets of the hop (= next frontier offsets)
+//   k_choose        variant per chunk (heuristic / forced / random / MLP),
+//                   appended to per-variant queues with wave-aggregated atomics
+//   k_expand<V,K>   persistent: pulls chunks of its queue and expands them
+//   k_expand_plain  one wave per entry in frontier order (the plain sampler)
+//
+// The reference runs all hops in one persistent warp loop whose output slots
+// and queue positions come from atomics (main.cu:21-93, functions.cuh:971-1011);
+// here every output position comes from the prefix sums, so the edge list is
+// identical whatever order chunks run in.
+#include "sample_kernels.hpp"
+#include "select.hpp"
+
+namespace csaw {
+
+// ------------------------------------------------------------------ count
+template <bool BIN>
+__global__ __launch_bounds__(256) void k_count(GraphDev G, const ReqDev* __restrict__ R, const int32_t* __restrict__ gs2req,
+                                               HopDev H) {
+  __shared__ uint32_t wsum[4];
+  __shared__ uint32_t hist[kNumBuckets];
+  __shared__ uint32_t bstart[kNumBuckets];
+  __shared__ uint32_t fsum[kNumBuckets];
+  __shared__ uint32_t fmax[kNumBuckets];
+  __shared__ uint32_t dmax[kNumBuckets];
+  __shared__ unsigned long long dsum[kNumBuckets];
+  __shared__ uint32_t nz_cnt[2];
+  __shared__ uint32_t chunk_base;
+  const int64_t n = *H.n_ptr;
+  const int64_t nblk = (n + kBlockEntries - 1) >> kBlockShift;
+  const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
+  for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const int64_t base = blk << kBlockShift;
+    if (BIN) {
+      for (int b = tid; b < kNumBuckets; b += 256) {
+        hist[b] = 0; fsum[b] = 0; fmax[b] = 0; dmax[b] = 0; dsum[b] = 0;
+      }
+      __syncthreads();
+    }
+    uint32_t k[4];
+    int bk[4];
+    uint32_t slot[4];
+    uint32_t tsum = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t e = base + tid * 4 + i;
+      k[i] = 0;
+      bk[i] = -1;
+      if (e < n) {
+        const int32_t v = H.fv[e];
+        const ReqDev& r = R[gs2req[H.fs[e]]];
+        const int32_t F = r.fanout[H.hop];
+        const int32_t d = G.deg[v];
+        k[i] = (uint32_t)entry_k(d, F, r.dedup);
+        if (BIN && k[i] > 0) {
+          bk[i] = entry_bucket(d, F, r.dedup, r.bias);
+          slot[i] = atomicAdd(&hist[bk[i]], 1u);
+          atomicAdd(&fsum[bk[i]], (uint32_t)F);
+          atomicMax(&fmax[bk[i]], (uint32_t)F);
+          atomicMax(&dmax[bk[i]], (uint32_t)d);
+          atomicAdd(&dsum[bk[i]], (unsigned long long)d);
+        }
+      }
+      tsum += k[i];
+    }
+    // block exclusive scan of per-thread sums
+    const uint32_t incl = wave_incl_scan_u32(tsum);
+    if (lane == kWave - 1) wsum[wid] = incl;
+    __syncthreads();
+    uint32_t pre = incl - tsum;
+    for (int w = 0; w < wid; ++w) pre += wsum[w];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t e = base + tid * 4 + i;
+      if (e < n) H.local_excl[e] = pre;
+      pre += k[i];
+    }
+    if (tid == 255) H.blk_excl[blk] = pre;  // block total, scanned later
+    if (BIN) {
+      // exclusive scan of the bucket histogram (threads 0..127 own one bucket)
+      uint32_t hv = 0, hs = 0;
+      if (tid < kNumBuckets) {
+        hv = hist[tid];
+        hs = wave_incl_scan_u32(hv);
+        if (lane == kWave - 1) nz_cnt[wid] = hs;  // wave totals
+      }
+      __syncthreads();
+      if (tid < kNumBuckets) bstart[tid] = hs - hv + (wid == 1 ? nz_cnt[0] : 0u);
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (bk[i] >= 0) H.order[base + bstart[bk[i]] + slot[i]] = (int32_t)(base + tid * 4 + i);
+      // chunk descriptors for non-empty buckets
+      uint64_t m = 0;
+      if (tid < kNumBuckets) m = ballot(hv > 0);
+      __syncthreads();
+      if (tid < kNumBuckets && lane == 0) nz_cnt[wid] = (uint32_t)__popcll(m);
+      __syncthreads();
+      if (tid == 0) chunk_base = atomicAdd(H.n_chunks, nz_cnt[0] + nz_cnt[1]);
+      __syncthreads();
+      if (tid < kNumBuckets && hv > 0) {
+        const uint32_t rank = (wid == 1 ? nz_cnt[0] : 0u) + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        const uint32_t ci = chunk_base + rank;
+        if (ci < H.chunk_cap) {
+          Chunk c;
+          c.start = (uint32_t)(base + bstart[tid]);
+          c.len = (uint16_t)hv;
+          c.bucket = (uint8_t)tid;
+          c.hop = (uint8_t)H.hop;
+          c.sum_f = fsum[tid];
+          c.max_d = dmax[tid];
+          c.sum_d = (float)dsum[tid];
+          c.key = (uint32_t)(blk * kNumBuckets + tid);
+          c.max_f = (uint8_t)fmax[tid];
+          H.chunks[ci] = c;
+        } else {
+          atomicOr(H.flags, 1u);
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// -------------------------------------------------------------- scans
+__global__ __launch_bounds__(256) void k_scan_blocks(HopDev H) {
+  __shared__ uint32_t wsum[4];
+  const int64_t n = *H.n_ptr;
+  const int64_t nblk = (n + kBlockEntries - 1) >> kBlockShift;
+  const int64_t ngrp = (nblk + 1023) >> 10;
+  const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
+  for (int64_t grp = blockIdx.x; grp < ngrp; grp += gridDim.x) {
+    const int64_t b0 = grp << 10;
+    uint32_t x[4], s = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t b = b0 + tid * 4 + i;
+      x[i] = b < nblk ? H.blk_excl[b] : 0u;
+      s += x[i];
+    }
+    const uint32_t incl = wave_incl_scan_u32(s);
+    if (lane == kWave - 1) wsum[wid] = incl;
+    __syncthreads();
+    uint32_t pre = incl - s;
+    for (int w = 0; w < wid; ++w) pre += wsum[w];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t b = b0 + tid * 4 + i;
+      if (b < nblk) H.blk_excl[b] = pre;
+      pre += x[i];
+    }
+    if (tid == 255) H.grp_excl[grp] = pre;  // group total
+    __syncthreads();
+  }
+}
+
+// one workgroup: exclusive scan of group totals; writes the hop total
+__global__ __launch_bounds__(256) void k_scan_groups(HopDev H, int64_t* __restrict__ total) {
+  __shared__ uint64_t wsum[4];
+  __shared__ uint64_t carry;
+  const int64_t n = *H.n_ptr;
+  const int64_t nblk = (n + kBlockEntries - 1) >> kBlockShift;
+  const int64_t ngrp = (nblk + 1023) >> 10;
+  const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
+  if (tid == 0) carry = 0;
+  __syncthreads();
+  for (int64_t g0 = 0; g0 < ngrp; g0 += 256) {
+    const int64_t gi = g0 + tid;
+    const uint64_t x = gi < ngrp ? H.grp_excl[gi] : 0ull;
+    const uint64_t incl = wave_incl_scan_u64(x);
+    if (lane == kWave - 1) wsum[wid] = incl;
+    __syncthreads();
+    uint64_t pre = carry + incl - x;
+    for (int w = 0; w < wid; ++w) pre += wsum[w];
+    if (gi < ngrp) H.grp_excl[gi] = pre;
+    __syncthreads();
+    if (tid == 255) carry = pre + x;
+    __syncthreads();
+  }
+  if (tid == 0) *total = (int64_t)carry;
+}
+
+// per-sample edge offsets for samples [0, gb] of this hop
+__global__ void k_offsets(HopDev H, const int64_t* __restrict__ total) {
+  const int64_t n = *H.n_ptr;
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g <= H.gb; g += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t x = H.fo[g];
+    H.eo[g] = x >= n ? *total
+                     : (int64_t)H.grp_excl[x >> kGroupShift] + H.blk_excl[x >> kBlockShift] + H.local_excl[x];
+  }
+}
+
+// --------------------------------------------------------- fills / resets
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ----------------------------------------------------------- expand plain
+template <int KIND>
+__global__ __launch_bounds__(256) void k_expand_plain(GraphDev G, csaw_graph_view hv, const ReqDev* __restrict__ R,
+                                                      const int32_t* __restrict__ gs2req, HopDev H) {
+  __shared__ uint64_t cdf[KIND == kInt ? 4 * kWaveIntCap : 1];
+  __shared__ uint32_t bms[4 * (kWaveBitmapCap / 32)];
+  const int64_t n = *H.n_ptr;
+  const int wid = threadIdx.x / kWave;
+  const int64_t nw = (int64_t)gridDim.x * 4;
+  for (int64_t e = blockIdx.x * 4 + wid; e < n; e += nw) {
+    const Entry en = load_entry(G, R, gs2req, H, e);
+    expand_entry_wave<KIND>(G, hv, H, en, KIND == kInt ? cdf + wid * kWaveIntCap : nullptr,
+                            bms + wid * (kWaveBitmapCap / 32));
+  }
+}
+
+// ------------------------------------------------------------- choose
+__device__ __forceinline__ bool variant_valid(int var, int kind, const Chunk& c) {
+  if (var == CSAW_VAR_THREAD) return c.max_f <= kThreadFanCap && (kind != kInt || c.max_d <= (uint32_t)kThreadIntCap);
+  return true;
+}
+
+__device__ int heuristic_variant(int kind, const Chunk& c);
+
+// Learned choice: argmin of the predicted log2 time over the valid variants.
+__device__ int mlp_choose(const ChooseArgs& A, const Chunk& c, int kind, const HopDev& H) {
+  float f[kMlpIn], o[kMlpOut];
+  chunk_features(c.len, c.sum_d, c.max_d, c.sum_f, c.max_f, kind, c.hop, c.bucket, *H.n_chunks, f);
+  mlp_forward1(A.mlp.w, f, o);
+  int best = -1;
+  float bv = 3.0e38f;
+  for (int v = 0; v < kMlpOut; ++v)
+    if (variant_valid(v, kind, c) && o[v] < bv) {
+      bv = o[v];
+      best = v;
+    }
+  return best >= 0 ? best : heuristic_variant(kind, c);
+}
+
+// The fixed heuristic (C-SAW degree thresholds SML_MID=32, MID_LRG=1024,
+// util.h:19-20, with the reference's default bitmap dedup, main.cu:26).
+__device__ int heuristic_variant(int kind, const Chunk& c) {
+  const uint32_t d = c.max_d;
+  if (d < 32 && variant_valid(CSAW_VAR_THREAD, kind, c)) return CSAW_VAR_THREAD;
+  if (d < 1024) return CSAW_VAR_WAVE_BITMAP;
+  return CSAW_VAR_BLOCK_BITMAP;
+}
+
+__global__ void k_choose(ChooseArgs A, HopDev H) {
+  const uint32_t nc = min(*H.n_chunks, H.chunk_cap);
+  for (uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x; ci - threadIdx.x < nc; ci += gridDim.x * blockDim.x) {
+    const bool act = ci < nc;
+    int q = -1;
+    if (act) {
+      const Chunk c = H.chunks[ci];
+      if (c.bucket == kBucketCopy) {
+        q = kQueueCopy;
+      } else {
+        const int kind = (c.bucket - 1) >> 5;
+        int var = heuristic_variant(kind, c);
+        if (A.sched == CSAW_SCHED_FORCED) {
+          if (variant_valid(A.variant, kind, c)) var = A.variant;
+        } else if (A.sched == CSAW_SCHED_RANDOM) {
+          const U4 o = philox4x32_10(c.key, (uint32_t)c.hop, 0u, kDomExplore << 28, (uint32_t)A.explore_key,
+                                     (uint32_t)(A.explore_key >> 32));
+          int cand = (int)(o.x % CSAW_NUM_VARIANTS);
+          for (int t = 0; t < CSAW_NUM_VARIANTS && !variant_valid(cand, kind, c); ++t) cand = (cand + 1) % CSAW_NUM_VARIANTS;
+          var = cand;
+        } else if (A.sched == CSAW_SCHED_MLP && A.mlp_w) {
+          var = mlp_choose(A, c, kind, H);
+        }
+        q = var * 3 + kind;
+        if (A.chosen) A.chosen[ci] = (uint8_t)var;
+      }
+    }
+    // wave-aggregated append to queue q
+    uint64_t active = ballot(act && q >= 0);
+    while (active) {
+      const int leader = __builtin_ctzll(active);
+      const int lq = readlane(q, leader);
+      const uint64_t m = ballot(act && q == lq);
+      uint32_t base = 0;
+      if (lane_id() == leader) base = atomicAdd(&A.qlen[lq], (uint32_t)__popcll(m));
+      base = readlane_u(base, leader);
+      if (act && q == lq) A.queues[(int64_t)lq * A.queue_cap + base + __popcll(m & ((1ull << lane_id()) - 1ull))] = ci;
+      active &= ~m;
+    }
+  }
+}
+
+// ------------------------------------------------------------- expand
+struct ChunkCursor {
+  uint32_t ci;     // chunk index
+  bool valid;
+};
+
+__device__ __forceinline__ ChunkCursor next_chunk(const ExpandArgs& A, int q, uint32_t* slot) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t i = atomicAdd(&A.qhead[q], 1u);
+    *slot = i < A.qlen[q] ? A.queues[(int64_t)q * A.queue_cap + i] : 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  ChunkCursor c;
+  c.ci = *slot;
+  c.valid = c.ci != 0xFFFFFFFFu;
+  return c;
+}
+
+__device__ __forceinline__ void log_chunk(const ExpandArgs& A, const HopDev& H, uint32_t ci, int var, uint64_t t0) {
+  if (A.log && threadIdx.x == 0) {
+    const uint64_t t1 = wall_clock64();
+    ChunkLog L;
+    L.key = H.chunks[ci].key;
+    L.hop = (uint8_t)H.hop;
+    L.variant = (uint8_t)var;
+    L.ticks = (uint32_t)(t1 - t0);
+    L.chunk = ci;
+    A.log[ci] = L;
+  }
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_expand_thread(GraphDev G, csaw_graph_view hv, const ReqDev* __restrict__ R,
+                                                       const int32_t* __restrict__ gs2req, HopDev H, ExpandArgs A) {
+  __shared__ uint32_t lds[kThreadIntCap * 256];
+  __shared__ uint32_t slot;
+  const int q = CSAW_VAR_THREAD * 3 + KIND;
+  for (;;) {
+    const ChunkCursor cc = next_chunk(A, q, &slot);
+    if (!cc.valid) break;
+    const uint64_t t0 = wall_clock64();
+    const Chunk c = H.chunks[cc.ci];
+    for (int k = threadIdx.x; k < c.len; k += 256) {
+      const Entry en = load_entry(G, R, gs2req, H, H.order[c.start + k]);
+      expand_entry_thread<KIND>(G, hv, H, en, lds + threadIdx.x, 256);
+    }
+    __syncthreads();
+    log_chunk(A, H, cc.ci, CSAW_VAR_THREAD, t0);
+  }
+}
+
+template <int KIND, bool BITMAP>
+__global__ __launch_bounds__(256) void k_expand_wave(GraphDev G, csaw_graph_view hv, const ReqDev* __restrict__ R,
+                                                     const int32_t* __restrict__ gs2req, HopDev H, ExpandArgs A) {
+  __shared__ uint64_t cdf[KIND == kInt ? 4 * kWaveIntCap : 1];
+  __shared__ uint32_t bms[BITMAP ? 4 * (kWaveBitmapCap / 32) : 1];
+  __shared__ uint32_t slot;
+  const int var = BITMAP ? CSAW_VAR_WAVE_BITMAP : CSAW_VAR_WAVE_HASH;
+  const int q = var * 3 + KIND;
+  const int wid = threadIdx.x / kWave;
+  for (;;) {
+    const ChunkCursor cc = next_chunk(A, q, &slot);
+    if (!cc.valid) break;
+    const uint64_t t0 = wall_clock64();
+    const Chunk c = H.chunks[cc.ci];
+    for (int k = wid; k < c.len; k += 4) {
+      const Entry en = load_entry(G, R, gs2req, H, H.order[c.start + k]);
+      expand_entry_wave<KIND>(G, hv, H, en, KIND == kInt ? cdf + wid * kWaveIntCap : nullptr,
+                              BITMAP ? bms + wid * (kWaveBitmapCap / 32) : nullptr);
+    }
+    __syncthreads();
+    log_chunk(A, H, cc.ci, var, t0);
+  }
+}
+
+// Workgroup granularity: 256 lanes build the integer CDF (up to kBlockIntCap
+// entries in LDS), wave 0 resolves the slots.
+template <int KIND, bool BITMAP>
+__global__ __launch_bounds__(256) void k_expand_block(GraphDev G, csaw_graph_view hv, const ReqDev* __restrict__ R,
+                                                      const int32_t* __restrict__ gs2req, HopDev H, ExpandArgs A) {
+  __shared__ uint64_t cdf[KIND == kInt ? kBlockIntCap : 1];
+  __shared__ uint32_t bms[BITMAP ? kBlockBitmapCap / 32 : 1];
+  __shared__ uint64_t wsum[4];
+  __shared__ uint64_t carry;
+  __shared__ uint32_t slot;
+  const int var = BITMAP ? CSAW_VAR_BLOCK_BITMAP : CSAW_VAR_BLOCK_HASH;
+  const int q = var * 3 + KIND;
+  const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
+  for (;;) {
+    const ChunkCursor cc = next_chunk(A, q, &slot);
+    if (!cc.valid) break;
+    const uint64_t t0 = wall_clock64();
+    const Chunk c = H.chunks[cc.ci];
+    for (int k = 0; k < c.len; ++k) {
+      const Entry en = load_entry(G, R, gs2req, H, H.order[c.start + k]);
+      if (en.d == 0 || en.F == 0) continue;
+      if (en.dedup && en.d <= en.F) {
+        for (int i = tid; i < en.d; i += 256) write_edge(G, hv, H, en, en.pos + i, i);
+        continue;
+      }
+      Picker pk;
+      bool build = !picker_global(G, en, KIND, kBlockIntCap, pk);
+      const bool use_bm = BITMAP && en.dedup && en.d <= kBlockBitmapCap;
+      if (build) {
+        // block-wide inclusive scan of the integer weights
+        if (tid == 0) carry = 0;
+        __syncthreads();
+        for (int i0 = 0; i0 < en.d; i0 += 256) {
+          const int i = i0 + tid;
+          uint64_t w = 0;
+          if (i < en.d) w = int_bias(G, hv, en.bias, G.adj[en.o + i]);
+          const uint64_t s = wave_incl_scan_u64(w);
+          if (lane == kWave - 1) wsum[wid] = s;
+          __syncthreads();
+          uint64_t pre = carry;
+          for (int w2 = 0; w2 < wid; ++w2) pre += wsum[w2];
+          if (i < en.d) cdf[i] = pre + s;
+          __syncthreads();
+          if (tid == 255) carry = pre + s;
+          __syncthreads();
+        }
+        pk.C = cdf;
+        pk.T = carry;
+        pk.mode = pk.T > 0 ? 1 : 0;
+      }
+      __syncthreads();
+      if (wid == 0) {
+        const int32_t res = wave_select(en, pk, use_bm ? bms : nullptr);
+        if (lane < en.F) write_edge(G, hv, H, en, en.pos + lane, res);
+      }
+      __syncthreads();
+    }
+    log_chunk(A, H, cc.ci, var, t0);
+  }
+}
+
+// copy-all chunks (d <= F without replacement): one lane per entry
+__global__ __launch_bounds__(256) void k_expand_copy(GraphDev G, csaw_graph_view hv, const ReqDev* __restrict__ R,
+                                                     const int32_t* __restrict__ gs2req, HopDev H, ExpandArgs A) {
+  __shared__ uint32_t slot;
+  for (;;) {
+    const ChunkCursor cc = next_chunk(A, kQueueCopy, &slot);
+    if (!cc.valid) break;
+    const Chunk c = H.chunks[cc.ci];
+    for (int k = threadIdx.x; k < c.len; k += 256) {
+      const Entry en = load_entry(G, R, gs2req, H, H.order[c.start + k]);
+      for (int32_t i = 0; i < en.d; ++i) write_edge(G, hv, H, en, en.pos + i, i);
+    }
+  }
+}
+
+// ------------------------------------------------------------ launchers
+static int grid_of(const LaunchCfg& L, int per_cu) {
+  return L.num_cu * per_cu;
+}
+
+int launch_count(const LaunchCfg& L, const GraphDev& G, const ReqDev* R, const int32_t* gs2req, const HopDev& H, bool bin,
+                 int64_t max_entries) {
+  int64_t nb = (max_entries + kBlockEntries - 1) / kBlockEntries;
+  int grid = (int)std::max<int64_t>(1, std::min<int64_t>(nb, grid_of(L, L.count_per_cu)));
+  if (bin) k_count<true><<<grid, 256, 0, L.stream>>>(G, R, gs2req, H);
+  else k_count<false><<<grid, 256, 0, L.stream>>>(G, R, gs2req, H);
+  CSAW_LAUNCH_CHECK();
+  return CSAW_OK;
+}
+
+int launch_scans(const LaunchCfg& L, const HopDev& H, int64_t* total, int64_t max_entries) {
+  int64_t nb = (max_entries + kBlockEntries - 1) / kBlockEntries;
+  int64_t ng = (nb + 1023) / 1024;
+  int grid = (int)std::max<int64_t>(1, std::min<int64_t>(ng, grid_of(L, 4)));
+  k_scan_blocks<<<grid, 256, 0, L.stream>>>(H);
+  CSAW_LAUNCH_CHECK();
+  k_scan_groups<<<1, 256, 0, L.stream>>>(H, total);
+  CSAW_LAUNCH_CHECK();
+  int64_t gs = H.gb + 1;
+  int g2 = (int)std::max<int64_t>(1, std::min<int64_t>((gs + 255) / 256, grid_of(L, 8)));
+  k_offsets<<<g2, 256, 0, L.stream>>>(H, total);
+  CSAW_LAUNCH_CHECK();
+  return CSAW_OK;
+}
+
+int launch_fill(const LaunchCfg& L, uint32_t* p, uint32_t v, int64_t n) {
+  if (n <= 0) return CSAW_OK;
+  int grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, grid_of(L, 4)));
+  k_fill_u32<<<grid, 256, 0, L.stream>>>(p, v, n);
+  CSAW_LAUNCH_CHECK();
+  return CSAW_OK;
+}
+
+int launch_expand_plain(const LaunchCfg& L, int kind, const GraphDev& G, const csaw_graph_view& hv, const ReqDev* R,
+                        const int32_t* gs2req, const HopDev& H, int64_t max_entries) {
+  int64_t nwg = (max_entries + 3) / 4;
+  int grid = (int)std::max<int64_t>(1, std::min<int64_t>(nwg, grid_of(L, L.expand_per_cu)));
+  if (L.grid_override > 0) grid = L.grid_override;
+  if (kind == kUniform) k_expand_plain<kUniform><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H);
+  else if (kind == kInt) k_expand_plain<kInt><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H);
+  else k_expand_plain<kF32><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H);
+  CSAW_LAUNCH_CHECK();
+  return CSAW_OK;
+}
+
+int launch_choose(const LaunchCfg& L, const ChooseArgs& A, const HopDev& H, int64_t max_chunks) {
+  int grid = (int)std::max<int64_t>(1, std::min<int64_t>((max_chunks + 255) / 256, grid_of(L, 8)));
+  k_choose<<<grid, 256, 0, L.stream>>>(A, H);
+  CSAW_LAUNCH_CHECK();
+  return CSAW_OK;
+}
+
+int launch_expand_chunks(const LaunchCfg& L, const GraphDev& G, const csaw_graph_view& hv, const ReqDev* R,
+                         const int32_t* gs2req, const HopDev& H, const ExpandArgs& A, const bool* kinds_present,
+                         TimingHook* th) {
+  auto go = [&](int which, auto kern, int per_cu) -> int {
+    int grid = grid_of(L, per_cu);
+    if (th) th->begin(which, L.stream);
+    kern<<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, A);
+    CSAW_LAUNCH_CHECK();
+    if (th) th->end(which, L.stream);
+    return CSAW_OK;
+  };
+  CSAW_TRY(go(kKernCopy, k_expand_copy, L.expand_per_cu));
+  for (int kind = 0; kind < 3; ++kind) {
+    if (!kinds_present[kind]) continue;
+    if (kind == kUniform) {
+      CSAW_TRY(go(kKernThread, k_expand_thread<kUniform>, L.thread_per_cu));
+      CSAW_TRY(go(kKernWave, k_expand_wave<kUniform, true>, L.expand_per_cu));
+      CSAW_TRY(go(kKernWave, k_expand_wave<kUniform, false>, L.expand_per_cu));
+      CSAW_TRY(go(kKernBlock, k_expand_block<kUniform, true>, L.block_per_cu));
+      CSAW_TRY(go(kKernBlock, k_expand_block<kUniform, false>, L.block_per_cu));
+    } else if (kind == kInt) {
+      CSAW_TRY(go(kKernThread, k_expand_thread<kInt>, L.thread_per_cu));
+      CSAW_TRY(go(kKernWave, k_expand_wave<kInt, true>, L.expand_per_cu));
+      CSAW_TRY(go(kKernWave, k_expand_wave<kInt, false>, L.expand_per_cu));
+      CSAW_TRY(go(kKernBlock, k_expand_block<kInt, true>, L.block_per_cu));
+      CSAW_TRY(go(kKernBlock, k_expand_block<kInt, false>, L.block_per_cu));
+    } else {
+      CSAW_TRY(go(kKernThread, k_expand_thread<kF32>, L.thread_per_cu));
+      CSAW_TRY(go(kKernWave, k_expand_wave<kF32, true>, L.expand_per_cu));
+      CSAW_TRY(go(kKernWave, k_expand_wave<kF32, false>, L.expand_per_cu));
+      CSAW_TRY(go(kKernBlock, k_expand_block<kF32, true>, L.block_per_cu));
+      CSAW_TRY(go(kKernBlock, k_expand_block<kF32, false>, L.block_per_cu));
+    }
+  }
+  return CSAW_OK;
+}
+
+}  // namespace csaw

@@ -1,0 +1,90 @@
+// Launch interfaces of the per-hop sampling kernels.
+#pragma once
+#include <algorithm>
+
+#include "common.hpp"
+#include "mlp.hpp"
+
+namespace csaw {
+
+constexpr int kNumQueues = 16;  // variant * 3 + kind, plus the copy-all queue
+constexpr int kQueueCopy = 15;
+
+enum KernKind : int {
+  kKernCount = 0,
+  kKernScan,
+  kKernChoose,
+  kKernCopy,
+  kKernThread,
+  kKernWave,
+  kKernBlock,
+  kKernPlain,
+  kKernWalk,
+  kKernMdrw,
+  kKernFill,
+  kNumKernKinds
+};
+static const char* const kKernNames[kNumKernKinds] = {"count", "scan", "choose", "expand_copy", "expand_thread",
+                                                      "expand_wave", "expand_block", "expand_plain", "walk", "mdrw",
+                                                      "fill"};
+
+struct ChunkLog {
+  uint32_t key;
+  uint32_t chunk;
+  uint32_t ticks;   // wall_clock64 ticks (100 MHz constant clock)
+  uint8_t hop;
+  uint8_t variant;
+  uint8_t pad[2];
+};
+
+struct LaunchCfg {
+  hipStream_t stream = nullptr;
+  int num_cu = 256;
+  int count_per_cu = 8;
+  int expand_per_cu = 8;   // WAVE / plain / copy workgroups per CU
+  int thread_per_cu = 4;   // THREAD workgroups per CU
+  int block_per_cu = 2;    // BLOCK workgroups per CU
+  int grid_override = 0;
+};
+
+struct ExpandArgs {
+  const uint32_t* queues;
+  int64_t queue_cap;
+  const uint32_t* qlen;
+  uint32_t* qhead;
+  ChunkLog* log;
+};
+
+struct ChooseArgs {
+  int32_t sched;
+  int32_t variant;
+  uint64_t explore_key;
+  uint32_t* qlen;
+  uint32_t* queues;
+  int64_t queue_cap;
+  uint8_t* chosen;   // per chunk variant (may be null)
+  MlpDev mlp;
+  const void* mlp_w; // non-null when a model is loaded
+};
+
+// Per-kernel-kind timing with hipEvents on the launch stream.
+struct TimingHook {
+  virtual void begin(int which, hipStream_t s) = 0;
+  virtual void end(int which, hipStream_t s) = 0;
+  virtual ~TimingHook() {}
+};
+
+__device__ int mlp_choose(const ChooseArgs& A, const Chunk& c, int kind, const HopDev& H);
+
+int launch_count(const LaunchCfg& L, const GraphDev& G, const ReqDev* R, const int32_t* gs2req, const HopDev& H,
+                 bool bin, int64_t max_entries);
+int launch_scans(const LaunchCfg& L, const HopDev& H, int64_t* total, int64_t max_entries);
+int launch_fill(const LaunchCfg& L, uint32_t* p, uint32_t v, int64_t n);
+int launch_expand_plain(const LaunchCfg& L, int kind, const GraphDev& G, const csaw_graph_view& hv, const ReqDev* R,
+                        const int32_t* gs2req, const HopDev& H, int64_t max_entries);
+int launch_choose(const LaunchCfg& L, const ChooseArgs& A, const HopDev& H, int64_t max_chunks);
+int launch_expand_chunks(const LaunchCfg& L, const GraphDev& G, const csaw_graph_view& hv, const ReqDev* R,
+                         const int32_t* gs2req, const HopDev& H, const ExpandArgs& A, const bool* kinds_present,
+                         TimingHook* th);
+
+}  // namespace csaw

@@ -1,0 +1,307 @@
+// Device-side inverse-transform selection for one frontier entry, in three
+// granularities (one lane, one wave64, one 256-lane workgroup) and two
+// taken-set representations (LDS bitmap or list).  Every path implements
+// the same canonical per-source selection, so all variants emit identical
+// edges:
+//   d == 0                      -> nothing            (main.cu:99-104)
+//   dedup && d <= F             -> all d neighbours   (functions.cuh:859-867)
+//   otherwise slot j = 0..F-1 in order; attempt a draws
+//     philox(key; (j<<16)|a, pos-in-sample, sample, DOM_SELECT|req|hop)
+//   and is accepted unless taken (bitmap_search, functions.cuh:45-70);
+//   after kReselect attempts (functions.cuh:844) the first untaken index
+//   scanning upward cyclically from the last draw is taken.
+// The ITS target: uniform i = mulhi64(t, d); integer biases an exact
+// uint64 inclusive CDF C and i = min{i: C_i > mulhi64(t, T)}; FP32 weights
+// the per-vertex sequential CDF c and i = min{i: c_i > fl32(u24 * T)}
+// (ITS + binary_search, functions.cuh:160-176, 11-43).
+#pragma once
+#include "common.hpp"
+
+namespace csaw {
+
+struct Entry {
+  int32_t v;
+  int32_t d;
+  int64_t o;
+  int32_t F;
+  int32_t dedup;
+  int32_t bias;   // csaw_bias
+  int32_t g;      // global sample id (written to the sample column)
+  int64_t pos;    // first output slot
+  uint32_t c1, c2, c3, k0, k1;
+};
+
+__device__ __forceinline__ Entry load_entry(const GraphDev& G, const ReqDev* __restrict__ R,
+                                            const int32_t* __restrict__ gs2req, const HopDev& H, int64_t e) {
+  Entry en;
+  en.v = H.fv[e];
+  en.g = H.fs[e];
+  const ReqDev& r = R[gs2req[en.g]];
+  en.F = r.fanout[H.hop];
+  en.dedup = r.dedup;
+  en.bias = r.bias;
+  en.o = G.off[en.v];
+  en.d = G.deg[en.v];
+  en.pos = (int64_t)H.grp_excl[e >> kGroupShift] + H.blk_excl[e >> kBlockShift] + H.local_excl[e];
+  en.c1 = (uint32_t)(e - H.fo[en.g]);
+  en.c2 = (uint32_t)(en.g - r.gbase);
+  en.c3 = select_c3(r.request_id, H.hop);
+  en.k0 = r.key_lo;
+  en.k1 = r.key_hi;
+  return en;
+}
+
+__device__ __forceinline__ U4 draw(const Entry& en, int j, int a) {
+  return philox4x32_10(((uint32_t)j << 16) | (uint32_t)a, en.c1, en.c2, en.c3, en.k0, en.k1);
+}
+
+__device__ __forceinline__ uint32_t int_bias(const GraphDev& G, const csaw_graph_view& hv, int32_t bias, int32_t u) {
+  return bias == CSAW_BIAS_USER ? EdgeBias(u, &hv) : (uint32_t)G.deg[u];
+}
+
+// ITS state of one entry.
+struct Picker {
+  int mode;            // 0 uniform, 1 integer CDF in LDS, 2 integer CDF in global, 3 FP32 CDF
+  int32_t d;
+  uint64_t T;
+  float Tf;
+  const uint64_t* C;   // LDS (mode 1) or global hub CDF (mode 2)
+  const float* Cf;
+
+  __device__ __forceinline__ int32_t pick(const U4& o) const {
+    const uint64_t t = philox_t64(o);
+    if (mode == 0) return (int32_t)mulhi64(t, (uint64_t)d);
+    if (mode == 3) {
+      const float x = __fmul_rn(philox_u24(o), Tf);
+      return (int32_t)upper_bound_dev<float>(Cf, d, x);
+    }
+    const uint64_t x = mulhi64(t, T);
+    return (int32_t)upper_bound_dev<uint64_t>(C, d, x);
+  }
+};
+
+// Picker for a bias that needs no per-entry build (uniform, FP32, hub cache).
+// Returns false when an integer CDF must be built by the caller.
+__device__ __forceinline__ bool picker_global(const GraphDev& G, const Entry& en, int kind, int cap, Picker& pk) {
+  pk.d = en.d;
+  pk.mode = 0;
+  if (kind == kUniform) return true;
+  if (kind == kF32) {
+    pk.Cf = G.cdf + en.o;
+    pk.Tf = pk.Cf[en.d - 1];
+    if (pk.Tf > 0.0f) pk.mode = 3;
+    return true;
+  }
+  if (en.d <= cap) return false;
+  const int32_t h = hub_find(G.hub_vid, G.n_hubs, en.v);
+  pk.C = G.hub_cdf[en.bias == CSAW_BIAS_USER ? 1 : 0] + G.hub_pool[h];
+  pk.T = pk.C[en.d - 1];
+  if (pk.T > 0) pk.mode = 2;
+  return true;
+}
+
+__device__ __forceinline__ void write_edge(const GraphDev& G, const csaw_graph_view& hv, const HopDev& H,
+                                           const Entry& en, int64_t p, int32_t idx) {
+  H.src[p] = en.v;
+  H.dst[p] = Update(&hv, G.adj[en.o + idx], en.v);
+  H.smp[p] = en.g;
+}
+
+__device__ __forceinline__ void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+// ---------------------------------------------------------------------------
+// Wave-level selection: lanes j < F hold slot j.  Slots resolve in order;
+// a colliding slot evaluates 64 further attempts at once (lane a -> attempt
+// a0+a) and takes the lowest free one, which is exactly the first free
+// attempt of the sequential definition.  Bitmap: LDS bitmap of d bits
+// (bm != nullptr); list: the already-resolved lanes' registers.
+// Returns the lane's resolved index (or -1 for lanes >= F).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool wave_taken_lane(int32_t ci, int32_t res, int jj, const uint32_t* bm) {
+  if (bm) return (bm[ci >> 5] >> (ci & 31)) & 1u;
+  bool hit = false;
+  for (int m = 0; m < jj; ++m) hit |= (ci == readlane(res, m));
+  return hit;
+}
+
+__device__ int32_t wave_select(const Entry& en, const Picker& pk, uint32_t* bm) {
+  const int lane = lane_id();
+  const int F = en.F;
+  const int32_t d = en.d;
+  int32_t res = -1;
+  if (!en.dedup) {
+    if (lane < F) res = pk.pick(draw(en, lane, 0));
+    return res;
+  }
+  if (bm) {
+    const int words = (d + 31) >> 5;
+    for (int i = lane; i < words; i += kWave) bm[i] = 0u;
+    wave_fence();
+  }
+  int32_t cand = 0;
+  if (lane < F) cand = pk.pick(draw(en, lane, 0));
+  for (int jj = 0; jj < F; ++jj) {
+    int32_t c = readlane(cand, jj);
+    bool taken = bm ? ((bm[c >> 5] >> (c & 31)) & 1u) : (ballot(res == c) != 0ull);
+    if (taken) {
+      int32_t chosen = -1, last = c;
+      for (int a0 = 1; a0 < kReselect; a0 += kWave) {
+        const int a = a0 + lane;
+        bool fr = false;
+        int32_t ci = 0;
+        if (a < kReselect) {
+          ci = pk.pick(draw(en, jj, a));
+          fr = !wave_taken_lane(ci, res, jj, bm);
+        }
+        const uint64_t m = ballot(fr);
+        if (m) {
+          chosen = readlane(ci, __builtin_ctzll(m));
+          break;
+        }
+        const int ll = kReselect - 1 - a0;
+        if (ll < kWave) last = readlane(ci, ll);
+      }
+      if (chosen < 0) {
+        for (int q = 0; q < d; q += kWave) {
+          const int32_t i = (int32_t)(((int64_t)last + 1 + q + lane) % d);
+          const bool fr = (q + lane < d) && !wave_taken_lane(i, res, jj, bm);
+          const uint64_t m = ballot(fr);
+          if (m) {
+            chosen = readlane(i, __builtin_ctzll(m));
+            break;
+          }
+        }
+      }
+      c = chosen;
+    }
+    if (bm) {
+      if (lane == 0) bm[c >> 5] |= 1u << (c & 31);
+      wave_fence();
+    }
+    if (lane == jj) res = c;
+  }
+  return res;
+}
+
+// Builds the integer CDF of entry en in LDS with one wave (d <= cap).
+__device__ __forceinline__ uint64_t wave_build_int_cdf(const GraphDev& G, const csaw_graph_view& hv, const Entry& en,
+                                                       uint64_t* C) {
+  const int lane = lane_id();
+  uint64_t acc = 0;
+  for (int i0 = 0; i0 < en.d; i0 += kWave) {
+    const int i = i0 + lane;
+    uint64_t w = 0;
+    if (i < en.d) w = int_bias(G, hv, en.bias, G.adj[en.o + i]);
+    const uint64_t s = wave_incl_scan_u64(w);
+    if (i < en.d) C[i] = acc + s;
+    acc += readlane_u64(s, kWave - 1);
+  }
+  wave_fence();
+  return acc;
+}
+
+// Copies all d <= F neighbours (the d <= NeighborSize path) with one wave.
+__device__ __forceinline__ void wave_copy_all(const GraphDev& G, const csaw_graph_view& hv, const HopDev& H,
+                                              const Entry& en) {
+  for (int i = lane_id(); i < en.d; i += kWave) write_edge(G, hv, H, en, en.pos + i, i);
+}
+
+// Full wave expansion of one entry.  cdf: LDS region of kWaveIntCap u64,
+// bm: LDS bitmap of kWaveBitmapCap bits (nullptr: list taken-set).
+template <int KIND>
+__device__ void expand_entry_wave(const GraphDev& G, const csaw_graph_view& hv, const HopDev& H, const Entry& en,
+                                  uint64_t* cdf, uint32_t* bm) {
+  if (en.d == 0 || en.F == 0) return;
+  if (en.dedup && en.d <= en.F) {
+    wave_copy_all(G, hv, H, en);
+    return;
+  }
+  Picker pk;
+  if (!picker_global(G, en, KIND, kWaveIntCap, pk)) {
+    pk.C = cdf;
+    pk.T = wave_build_int_cdf(G, hv, en, cdf);
+    pk.mode = pk.T > 0 ? 1 : 0;
+  }
+  const int32_t res = wave_select(en, pk, (bm && en.d <= kWaveBitmapCap) ? bm : nullptr);
+  if (lane_id() < en.F) write_edge(G, hv, H, en, en.pos + lane_id(), res);
+}
+
+// ---------------------------------------------------------------------------
+// Lane-level selection (THREAD variant).  Valid for F <= kThreadFanCap and,
+// for integer biases, d <= kThreadIntCap (weights cached in LDS, transposed
+// per lane).  Taken set: a 64-bit mask when d <= 64, else a per-lane LDS list.
+// ---------------------------------------------------------------------------
+template <int KIND>
+__device__ void expand_entry_thread(const GraphDev& G, const csaw_graph_view& hv, const HopDev& H, const Entry& en,
+                                    uint32_t* lds, int stride) {
+  if (en.d == 0 || en.F == 0) return;
+  const int32_t d = en.d;
+  if (en.dedup && d <= en.F) {
+    for (int32_t i = 0; i < d; ++i) write_edge(G, hv, H, en, en.pos + i, i);
+    return;
+  }
+  // picker
+  int mode = 0;
+  uint64_t T = 0;
+  float Tf = 0.f;
+  const float* Cf = nullptr;
+  if (KIND == kF32) {
+    Cf = G.cdf + en.o;
+    Tf = Cf[d - 1];
+    if (Tf > 0.0f) mode = 3;
+  } else if (KIND == kInt) {
+    for (int32_t i = 0; i < d; ++i) {
+      const uint32_t w = int_bias(G, hv, en.bias, G.adj[en.o + i]);
+      lds[i * stride] = w;
+      T += w;
+    }
+    if (T > 0) mode = 1;
+  }
+  auto pick = [&](const U4& o) -> int32_t {
+    const uint64_t t = philox_t64(o);
+    if (mode == 0) return (int32_t)mulhi64(t, (uint64_t)d);
+    if (mode == 3) return (int32_t)upper_bound_dev<float>(Cf, d, __fmul_rn(philox_u24(o), Tf));
+    const uint64_t x = mulhi64(t, T);
+    uint64_t acc = 0;
+    int32_t i = 0;
+    for (; i < d - 1; ++i) {
+      acc += lds[i * stride];
+      if (acc > x) break;
+    }
+    return i;
+  };
+  if (!en.dedup) {
+    for (int j = 0; j < en.F; ++j) write_edge(G, hv, H, en, en.pos + j, pick(draw(en, j, 0)));
+    return;
+  }
+  const bool use_mask = d <= 64;
+  uint64_t mask = 0;
+  int n_taken = 0;
+  auto taken = [&](int32_t i) -> bool {
+    if (use_mask) return (mask >> i) & 1ull;
+    for (int m = 0; m < n_taken; ++m)
+      if ((int32_t)lds[m * stride] == i) return true;
+    return false;
+  };
+  for (int j = 0; j < en.F; ++j) {
+    int32_t i = 0;
+    bool found = false;
+    for (int a = 0; a < kReselect; ++a) {
+      i = pick(draw(en, j, a));
+      if (!taken(i)) {
+        found = true;
+        break;
+      }
+    }
+    if (!found) {
+      do {
+        i = (i + 1 == d) ? 0 : i + 1;
+      } while (taken(i));
+    }
+    if (use_mask) mask |= 1ull << i;
+    else lds[(n_taken++) * stride] = (uint32_t)i;
+    write_edge(G, hv, H, en, en.pos + j, i);
+  }
+}
+
+}  // namespace csaw

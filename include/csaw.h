@@ -1,0 +1,254 @@
+/*
+ * csaw.h -- C ABI of the MI355X-native C-SAW sampling / random-walk engine.
+ *
+ * Plain C: pointers, sizes, status codes; no HIP or torch types.  Every
+ * function returns a csaw_status (0 = OK) unless stated otherwise and never
+ * calls exit().  Host pointers are read synchronously.  Device work runs on
+ * the library's own stream unless a stream is passed (void* = hipStream_t).
+ *
+ * Reference surfaces replaced (paths relative to the C-SAW reference root):
+ *  - struct arguments Sampler(beg, csr, n_blocks, n_threads, n_subgraph,
+ *    FrontierSize, NeighborSize, Depth, args, rank)   non-stream/sampler.cuh:1-8,
+ *    non-stream/main.cu:196-308, streaming/streaming_sampling.cu:427-728
+ *  - class Sampling / Si (samples, FrontierSize, NeighborSize, Depth, output
+ *    edge list Si{vertex,edge,start})                 non-stream/sample_class.cuh:76-90,146-185
+ *  - graph<...>(beg_file, csr_file, weight_file)      non-stream/graph.hpp:4-111
+ *  - gpu_graph(ginst)                                 non-stream/gpu_graph.cuh:10-65
+ *  - VertexBias / EdgeBias / Update hooks             non-stream/api.cuh:4-27
+ *    (device side: c-saw_amd/csrc/csaw_api.h; selected here by csaw_bias)
+ */
+#ifndef CSAW_H
+#define CSAW_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CSAW_MAX_HOPS 8
+#define CSAW_MAX_FANOUT 64
+
+/* Status codes (replace H_ERR/HRR print-and-exit, util.h:5-16, herror.h:2-10). */
+typedef enum csaw_status {
+  CSAW_OK = 0,
+  CSAW_ERR_INVALID_ARG = 1,
+  CSAW_ERR_INVALID_GRAPH = 2,
+  CSAW_ERR_OOM = 3,
+  CSAW_ERR_BUDGET_TOO_SMALL = 4,
+  CSAW_ERR_HIP = 5,
+  CSAW_ERR_CAPACITY = 6,
+  CSAW_ERR_NO_DEVICE = 7,
+  CSAW_ERR_UNSUPPORTED = 8,
+  CSAW_ERR_IO = 9
+} csaw_status;
+
+/* Bias policy: which EdgeBias hook the ITS select uses (api.cuh:13-20). */
+typedef enum csaw_bias {
+  CSAW_BIAS_UNIFORM = 0, /* EdgeBias = 1                                   */
+  CSAW_BIAS_DEGREE = 1,  /* EdgeBias = degree of the neighbour (integer)   */
+  CSAW_BIAS_WEIGHT = 2,  /* FP32 edge weight, sequential CDF in CSR order  */
+  CSAW_BIAS_USER = 3     /* the compiled-in user hooks of csaw_api.h       */
+} csaw_bias;
+
+/* Kernel variants (granularity x taken-set representation).  Every variant
+ * returns bit-identical edge lists; they differ only in speed. */
+typedef enum csaw_variant {
+  CSAW_VAR_THREAD = 0,      /* one lane per frontier entry                  */
+  CSAW_VAR_WAVE_BITMAP = 1, /* one wave64 per entry, LDS bitmap taken-set    */
+  CSAW_VAR_WAVE_HASH = 2,   /* one wave64 per entry, register/LDS list set   */
+  CSAW_VAR_BLOCK_BITMAP = 3,/* one 256-lane workgroup per entry, bitmap      */
+  CSAW_VAR_BLOCK_HASH = 4,  /* one 256-lane workgroup per entry, list set    */
+  CSAW_NUM_VARIANTS = 5
+} csaw_variant;
+
+typedef enum csaw_sched {
+  CSAW_SCHED_HEURISTIC = 0, /* C-SAW degree thresholds (util.h:19-20)         */
+  CSAW_SCHED_MLP = 1,       /* learned cost model, heuristic as fallback      */
+  CSAW_SCHED_FORCED = 2,    /* one variant for every chunk it is valid for    */
+  CSAW_SCHED_RANDOM = 3     /* random valid variant per chunk (training data) */
+} csaw_sched;
+
+typedef enum csaw_mode {
+  CSAW_MODE_PLAIN = 0,  /* one request at a time, host-synchronised hops     */
+  CSAW_MODE_BATCHED = 1 /* all requests per launch, device-side sizes        */
+} csaw_mode;
+
+/* Human-readable text of the last error on this thread. */
+const char *csaw_last_error(void);
+
+/* ---------------------------------------------------------------- graph */
+typedef struct csaw_graph csaw_graph;
+
+/* Uploads a host CSR.  offsets: V+1 entries, int32 or int64 per
+ * offsets_are_i64; indices: E int32 neighbour ids; weights: E floats or NULL.
+ * Validates monotone offsets, offsets[0]==0, offsets[V]==E, ids in [0,V). */
+int csaw_graph_create(const void *offsets, int offsets_are_i64, const int32_t *indices,
+                      const float *weights_or_null, int64_t V, int64_t E, csaw_graph **out);
+/* Generates an RMAT graph on the GPU (ef*2^scale Philox-drawn pairs,
+ * symmetrised, self loops and duplicates removed, sorted lists) and, when
+ * with_weights, FP32 weights uniform in (0,1] per CSR entry. */
+int csaw_graph_create_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t key,
+                           int with_weights, csaw_graph **out);
+/* Reads the reference's on-disk CSR (beg.bin: V+1 int64 offsets, csr.bin:
+ * E int64 ids; graph.hpp:18-74) and uploads it; weight_file may be NULL. */
+int csaw_graph_load_bin(const char *beg_file, const char *csr_file, const char *weight_file,
+                        csaw_graph **out);
+int csaw_graph_info(const csaw_graph *g, int64_t *V, int64_t *E, int64_t *max_degree);
+/* Copies the CSR back to host buffers (any may be NULL). */
+int csaw_graph_copy_csr(const csaw_graph *g, int64_t *offsets, int32_t *indices, float *weights);
+int csaw_graph_destroy(csaw_graph *g);
+
+/* Seeds A.8: seed_i = mulhi64(philox(key; i, request), V), written to host. */
+int csaw_seeds(const csaw_graph *g, uint64_t key, uint32_t request, int64_t n, int32_t *out);
+
+/* ------------------------------------------------------------- sampling */
+/* The Sampling-class parameter surface (sample_class.cuh:146-185 and the
+ * Sampler() argument list, sampler.cuh:3-8). */
+typedef struct csaw_params {
+  int32_t samples;       /* n_subgraph                                     */
+  int32_t frontier_size; /* FrontierSize (1: neighbour sampling / walk;    */
+                         /*  >1: multi-dimensional random walk)            */
+  int32_t neighbor_size; /* NeighborSize (fanout when fanouts == NULL)     */
+  int32_t depth;         /* Depth                                          */
+  const int32_t *fanouts;/* optional per-hop fanouts, `depth` entries      */
+  int32_t bias;          /* csaw_bias                                      */
+  int32_t dedup;         /* 1: no repeated neighbour per source; 0: with   */
+                         /*  replacement                                   */
+  uint64_t seed;         /* Philox key                                     */
+  uint32_t request_id;   /* counter word; keeps batched == plain           */
+  int32_t sched;         /* csaw_sched                                     */
+  int32_t variant;       /* csaw_variant for CSAW_SCHED_FORCED             */
+  int32_t n_blocks;      /* launch overrides (0 = automatic)               */
+  int32_t n_threads;
+} csaw_params;
+
+typedef struct csaw_stats {
+  int64_t edges;          /* sampled edges (or moved walk steps)           */
+  double kernel_ms;       /* hipEvent time of the device work              */
+  double algo_bytes;      /* algorithmic bytes of the expansion trace      */
+  int64_t expansions;     /* frontier entries expanded                     */
+  int64_t variant_chunks[CSAW_NUM_VARIANTS]; /* chunks run per variant     */
+} csaw_stats;
+
+/* Output edge list Si, hop-major: hop h, then sample, then parent position,
+ * then slot.  hop_offsets has depth*(samples+1) entries (absolute indices). */
+typedef struct csaw_edges csaw_edges;
+
+/* Sampler(): seeds are host int32 of samples*frontier_size entries. */
+int csaw_sample(csaw_graph *g, const csaw_params *p, const int32_t *seeds, csaw_edges **out,
+                csaw_stats *stats);
+int csaw_edges_info(const csaw_edges *e, int64_t *n_edges, int32_t *depth, int32_t *samples);
+int csaw_edges_copy(const csaw_edges *e, int32_t *src, int32_t *dst, int32_t *sample,
+                    int64_t *hop_offsets);
+void csaw_edges_free(csaw_edges *e);
+
+/* Random walk: walks[w*(len+1)+t], -1 after a degree-0 vertex. */
+int csaw_random_walk(csaw_graph *g, int32_t bias, uint64_t seed, uint32_t request_id,
+                     const int32_t *seeds, int64_t n_walkers, int32_t len, int32_t *walks,
+                     csaw_stats *stats);
+
+/* Launch shapes (workgroups per CU per kernel family), chosen by
+ * csaw_autotune.  Replaces the fixed <<<n_blocks, n_threads>>> of the
+ * reference's run.sh:2 (400 x 128). */
+typedef struct csaw_tuning {
+  int32_t count_per_cu;
+  int32_t expand_per_cu;
+  int32_t thread_per_cu;
+  int32_t block_per_cu;
+  int32_t grid_override; /* fixed grid for the plain expand kernel (0: auto) */
+} csaw_tuning;
+
+/* Learned cost model (16 -> 64 -> 64 -> CSAW_NUM_VARIANTS, BF16 weights,
+ * FP32 accumulate); trained and evaluated by the library's HIP kernels. */
+typedef struct csaw_mlp csaw_mlp;
+int csaw_mlp_num_params(void);
+int csaw_mlp_create(uint64_t init_key, csaw_mlp **out);
+int csaw_mlp_destroy(csaw_mlp *m);
+/* FP32 master parameters (csaw_mlp_num_params floats). */
+int csaw_mlp_get_params(const csaw_mlp *m, float *params);
+int csaw_mlp_set_params(csaw_mlp *m, const float *params);
+/* Forward on the GPU: X[n][16] -> Y[n][CSAW_NUM_VARIANTS]. */
+int csaw_mlp_forward(csaw_mlp *m, const float *X, int64_t n, float *Y);
+/* Masked mean-squared-error loss and its gradient with respect to the
+ * (BF16-rounded) parameters, on the GPU.  T, M: [n][CSAW_NUM_VARIANTS]. */
+int csaw_mlp_grad(csaw_mlp *m, const float *X, const float *T, const float *M, int64_t n, float *loss,
+                  float *grad);
+/* Minibatch SGD on the GPU (forward, backward and update kernels). */
+int csaw_mlp_train(csaw_mlp *m, const float *X, const float *T, const float *M, int64_t n, int32_t epochs,
+                   int32_t batch, float lr, uint64_t shuffle_key, float *final_loss);
+
+/* Multi-dimensional random walk (FrontierSize > 1; check_layer, main.cu:95-194):
+ * seeds has samples*frontier_size entries; src/dst hold samples*depth
+ * edges; offsets[samples+1] delimit each sample's edges. */
+int csaw_mdrw(csaw_graph *g, uint64_t seed, uint32_t request_id, int32_t samples, int32_t frontier_size,
+              int32_t depth, const int32_t *seeds, int32_t *src, int32_t *dst, int64_t *offsets,
+              csaw_stats *stats);
+
+/* Reusable walk plan: buffers allocated once, seeds resident on the device. */
+typedef struct csaw_walk_plan csaw_walk_plan;
+int csaw_walk_plan_create(csaw_graph *g, int32_t bias, uint64_t seed, uint32_t request_id,
+                          const int32_t *seeds, int64_t n_walkers, int32_t len, csaw_walk_plan **out);
+int csaw_walk_plan_run(csaw_walk_plan *p, void *stream, int sync);
+int csaw_walk_plan_result(const csaw_walk_plan *p, int32_t *walks, int64_t *steps, double *kernel_ms);
+int csaw_walk_plan_destroy(csaw_walk_plan *p);
+
+/* ------------------------------------------------ plans (batched, reusable) */
+typedef struct csaw_request {
+  int32_t samples;
+  int32_t depth;
+  int32_t fanouts[CSAW_MAX_HOPS];
+  int32_t bias;
+  int32_t dedup;
+  uint64_t seed;
+  uint32_t request_id;
+  const int32_t *seeds; /* host, `samples` entries */
+} csaw_request;
+
+typedef struct csaw_plan_opts {
+  int32_t mode;      /* csaw_mode                                          */
+  int32_t sched;     /* csaw_sched                                         */
+  int32_t variant;   /* forced variant                                     */
+  int32_t use_graph; /* capture the hop sequence in a hipGraph (batched)   */
+  int32_t timing;    /* record per-kernel hipEvent times                   */
+  int32_t log_chunks;/* record per-chunk timings (cost-model data)         */
+  uint64_t explore_key; /* key of CSAW_SCHED_RANDOM                        */
+  const struct csaw_mlp *mlp; /* cost model for CSAW_SCHED_MLP (or NULL)   */
+  const struct csaw_tuning *tuning; /* launch shapes (or NULL: defaults)   */
+} csaw_plan_opts;
+
+typedef struct csaw_plan csaw_plan;
+int csaw_plan_create(csaw_graph *g, const csaw_request *reqs, int32_t n_req,
+                     const csaw_plan_opts *opts, csaw_plan **out);
+/* Runs every request.  stream: hipStream_t or NULL for the plan's stream.
+ * Blocks until the device work is done when sync != 0. */
+int csaw_plan_run(csaw_plan *p, void *stream, int sync);
+int csaw_plan_stats(const csaw_plan *p, csaw_stats *stats);
+int csaw_plan_request_info(const csaw_plan *p, int32_t req, int64_t *n_edges);
+int csaw_plan_copy(const csaw_plan *p, int32_t req, int32_t *src, int32_t *dst, int32_t *sample,
+                   int64_t *hop_offsets);
+/* Average device time per launch of each kernel kind over the runs since
+ * the last reset (timing != 0).  names: NUL-separated list; returns count. */
+int csaw_plan_kernel_times(const csaw_plan *p, char *names, size_t names_len, double *avg_ms,
+                           int64_t *launches, int32_t max_kinds, int32_t *n_kinds);
+int csaw_plan_reset_times(csaw_plan *p);
+int csaw_plan_destroy(csaw_plan *p);
+/* Per-chunk cost-model data of the last run (log_chunks != 0): features
+ * [n][16], the variant that ran, log2 of its device ticks, a chunk key that
+ * is stable across runs of the same plan (hop << 32 | block*128 + bucket). */
+int csaw_plan_chunk_log(const csaw_plan *p, float *features, int32_t *variant, float *log2_ticks,
+                        uint64_t *keys, int64_t max_n, int64_t *n);
+/* Tries launch shapes on the given requests and returns the fastest. */
+int csaw_autotune(csaw_graph *g, const csaw_request *reqs, int32_t n_req, const csaw_plan_opts *base,
+                  csaw_tuning *best, double *best_ms);
+
+/* C4 request mix: n requests, seed counts log-uniform in [lo, hi], fanouts
+ * {25,10} or {15,10,5}, bias uniform / degree / weight with p = 1/3 each,
+ * all drawn from Philox(mix_key).  Writes the request headers (seeds are
+ * generated with csaw_seeds by the caller); returns total seeds. */
+int64_t csaw_request_mix(uint64_t mix_key, int32_t n, int32_t lo, int32_t hi, csaw_request *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

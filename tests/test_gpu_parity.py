@@ -1,0 +1,221 @@
+"""GPU parity: the HIP kernels against the CPU oracle, bit for bit.
+
+Every variant (thread / wave / workgroup granularity, bitmap / list taken
+set), the plain and batched executors and the hipGraph replay must emit the
+oracle's edge lists exactly (integer and index work: bit-exact)."""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+KEYS = ("src", "dst", "smp", "hop_off")
+
+
+def same(a, b, what=""):
+    for k in KEYS:
+        assert np.array_equal(a[k], b[k]), f"{what}: {k} differs"
+
+
+@pytest.fixture(scope="module")
+def g14(C, O):
+    g = C.Graph.rmat(14, 8, weights=True)
+    off, adj, w = g.csr(weights=True)
+    return g, O.OracleGraph(off, adj, weights=w), off, adj, w
+
+
+def test_rmat_matches_oracle(C, O):
+    for scale, ef in ((10, 4), (13, 8)):
+        g = C.Graph.rmat(scale, ef, weights=True)
+        off, adj, w = g.csr(weights=True)
+        off_o, adj_o = O.rmat(scale, ef)
+        assert np.array_equal(off, off_o) and np.array_equal(adj, adj_o)
+        assert np.array_equal(w, O.weights(len(adj)))
+        assert np.array_equal(g.seeds(500, request=2), O.seeds(g.V, 500, request=2))
+
+
+def test_rmat16_digest_on_gpu(C):
+    g = C.Graph.rmat(16, 8)
+    off, adj = g.csr()
+    want = json.load(open(os.path.join(HERE, "golden", "rmat16.json")))
+    assert hashlib.sha256(off.tobytes() + adj.tobytes()).hexdigest() == want["sha256"]
+    assert g.max_degree == want["max_degree"]
+
+
+def test_config1_plain_uniform(C, O):
+    # RMAT-16 ef 8; 1000 samples; FrontierSize 1, NeighborSize 2, Depth 2; uniform; bitmap; seed 42
+    g = C.Graph.rmat(16, 8)
+    off, adj = g.csr()
+    og = O.OracleGraph(off, adj)
+    seeds = g.seeds(1000)
+    want = O.sample(og, seeds, [2, 2], bias=O.BIAS_UNIFORM)
+    same(C.sample(g, seeds, [2, 2], bias=C.BIAS_UNIFORM), want, "plain")
+    same(C.sample(g, seeds, [2, 2], bias=C.BIAS_UNIFORM, sched=C.SCHED_FORCED, variant=C.VAR_WAVE_BITMAP), want,
+         "wave bitmap")
+
+
+@pytest.mark.parametrize("bias", [0, 1, 2, 3])
+def test_all_variants_match_oracle(C, O, g14, bias):
+    g, og, off, adj, w = g14
+    seeds = g.seeds(700, request=bias)
+    obias = bias if bias != 3 else O.BIAS_USER
+    want = O.sample(og, seeds, [12, 6, 3], bias=obias)
+    same(C.sample(g, seeds, [12, 6, 3], bias=bias), want, "plain")
+    for v in range(C.NUM_VARIANTS):
+        same(C.sample(g, seeds, [12, 6, 3], bias=bias, sched=C.SCHED_FORCED, variant=v), want, f"variant {v}")
+    same(C.sample(g, seeds, [12, 6, 3], bias=bias, sched=C.SCHED_RANDOM), want, "random")
+
+
+@pytest.mark.parametrize("bias", [0, 1, 2])
+def test_no_dedup_matches_oracle(C, O, g14, bias):
+    g, og, *_ = g14
+    seeds = g.seeds(500, request=7)
+    want = O.sample(og, seeds, [5, 4], bias=bias, dedup=0)
+    same(C.sample(g, seeds, [5, 4], bias=bias, dedup=0), want, "plain")
+    for v in range(C.NUM_VARIANTS):
+        same(C.sample(g, seeds, [5, 4], bias=bias, dedup=0, sched=C.SCHED_FORCED, variant=v), want, f"v{v}")
+
+
+def test_config2_degree_bias(C, O):
+    # RMAT-20 ef 5; 8192 seeds; fanouts {25,10}; degree bias; no repeats per source
+    g = C.Graph.rmat(20, 5)
+    off, adj = g.csr()
+    og = O.OracleGraph(off, adj)
+    seeds = g.seeds(8192)
+    want = O.sample(og, seeds, [25, 10], bias=O.BIAS_DEGREE, nthreads=8)
+    same(C.sample(g, seeds, [25, 10], bias=C.BIAS_DEGREE), want, "plain")
+    same(C.sample(g, seeds, [25, 10], bias=C.BIAS_DEGREE, sched=C.SCHED_RANDOM), want, "random variants")
+
+
+def star_with_hub(O, hub_deg, extra):
+    """vertex 0 has hub_deg neighbours with varied degrees (hub-cache path)."""
+    rng = np.random.default_rng(hub_deg)
+    edges = []
+    nxt = hub_deg + 1
+    for i in range(1, hub_deg + 1):
+        edges.append((0, i))
+        for _ in range(int(rng.integers(0, extra))):
+            edges.append((i, nxt))
+            nxt += 1
+    und = sorted(edges + [(b, a) for a, b in edges])
+    V = nxt
+    src = np.array([a for a, _ in und], np.int64)
+    adj = np.array([b for _, b in und], np.int32)
+    off = np.zeros(V + 1, np.int64)
+    np.add.at(off, src + 1, 1)
+    return np.cumsum(off), adj
+
+
+@pytest.mark.parametrize("hub_deg", [1500, 9000, 20000])
+def test_hub_vertices(C, O, hub_deg):
+    off, adj = star_with_hub(O, hub_deg, 4)
+    w = O.weights(len(adj), key=9)
+    g = C.Graph.from_csr(off, adj, w)
+    og = O.OracleGraph(off, adj, weights=w)
+    seeds = np.zeros(64, np.int32)
+    for bias in (0, 1, 2):
+        want = O.sample(og, seeds, [64, 2], bias=bias)
+        same(C.sample(g, seeds, [64, 2], bias=bias), want, f"plain b{bias}")
+        for v in range(C.NUM_VARIANTS):
+            same(C.sample(g, seeds, [64, 2], bias=bias, sched=C.SCHED_FORCED, variant=v), want, f"b{bias} v{v}")
+
+
+def test_edge_cases(C, O):
+    # zero-degree seeds, d == F, d == F + 1, isolated vertices, asymmetric
+    # graph with zero-degree neighbours (all-zero degree weights -> uniform)
+    off = np.array([0, 3, 5, 5, 6, 7, 7, 9], np.int64)
+    adj = np.array([1, 2, 6, 0, 3, 4, 5, 2, 5], np.int32)
+    w = np.array([0.5, 0.25, 1.0, 1.0, 0.5, 1.0, 1.0, 0.0, 0.0], np.float32)
+    g = C.Graph.from_csr(off, adj, w)
+    og = O.OracleGraph(off, adj, weights=w)
+    seeds = np.array([0, 1, 2, 3, 4, 5, 6, 0, 6, 2], np.int32)
+    for bias in (0, 1, 2):
+        for fan in ([1], [2], [3], [2, 2, 2], [64, 1]):
+            for dedup in (0, 1):
+                want = O.sample(og, seeds, fan, bias=bias, dedup=dedup)
+                same(C.sample(g, seeds, fan, bias=bias, dedup=dedup), want, f"{bias} {fan} {dedup}")
+                same(C.sample(g, seeds, fan, bias=bias, dedup=dedup, sched=C.SCHED_RANDOM), want, "random")
+
+
+def test_empty_requests(C, O, g14):
+    g, og, *_ = g14
+    r = C.sample(g, np.zeros(0, np.int32), [3, 2])
+    assert len(r["src"]) == 0 and r["hop_off"].shape == (2, 1)
+    r = C.sample(g, g.seeds(10), [])
+    assert len(r["src"]) == 0
+    r = C.sample(g, g.seeds(10), [0, 3])
+    assert len(r["src"]) == 0
+
+
+def test_invalid_arguments(C, g14):
+    g, *_ = g14
+    with pytest.raises(C.CsawError):
+        C.sample(g, np.array([g.V], np.int32), [2])
+    with pytest.raises(C.CsawError):
+        C.sample(g, np.array([0], np.int32), [65])
+    g2 = C.Graph.rmat(8, 4)
+    with pytest.raises(C.CsawError):
+        C.sample(g2, np.array([0], np.int32), [2], bias=C.BIAS_WEIGHT)
+    with pytest.raises(C.CsawError):
+        C.Graph.from_csr(np.array([0, 2, 1], np.int64), np.array([1, 0], np.int32))
+    with pytest.raises(C.CsawError):
+        C.Graph.from_csr(np.array([0, 1, 2], np.int64), np.array([1, 5], np.int32))
+
+
+def test_batched_equals_plain_and_oracle(C, O, g14):
+    g, og, *_ = g14
+    specs = []
+    rng = np.random.default_rng(3)
+    for i in range(40):
+        fan = [25, 10] if i % 2 else [15, 10, 5]
+        specs.append({"seeds": g.seeds(int(rng.integers(1, 200)), request=i), "fanouts": fan, "bias": i % 3,
+                      "request": i})
+    plain = C.Plan(g, specs, mode=C.MODE_PLAIN)
+    plain.run()
+    for sched in (C.SCHED_HEURISTIC, C.SCHED_RANDOM):
+        bat = C.Plan(g, specs, mode=C.MODE_BATCHED, sched=sched, explore_key=11)
+        bat.run()
+        for i, s in enumerate(specs):
+            want = O.sample(og, s["seeds"], s["fanouts"], bias=s["bias"], request=i)
+            same(plain.result(i), want, f"plain req {i}")
+            same(bat.result(i), want, f"batched req {i}")
+
+
+def test_hipgraph_replays(C, O, g14):
+    g, og, *_ = g14
+    specs = [{"seeds": g.seeds(300, request=i), "fanouts": [10, 5], "bias": i % 3, "request": i} for i in range(6)]
+    want = [O.sample(og, s["seeds"], s["fanouts"], bias=s["bias"], request=i) for i, s in enumerate(specs)]
+    p = C.Plan(g, specs, mode=C.MODE_BATCHED, use_graph=True)
+    for rep in range(4):
+        p.run()
+        for i in range(len(specs)):
+            same(p.result(i), want[i], f"replay {rep} req {i}")
+        # eager work between replays
+        C.sample(g, specs[0]["seeds"], [3])
+
+
+@pytest.mark.parametrize("bias", [0, 1, 2])
+def test_walk_matches_oracle(C, O, g14, bias):
+    g, og, *_ = g14
+    seeds = g.seeds(3000, request=5)
+    W, steps = C.random_walk(g, seeds, 40, bias=bias)
+    Wo, so = O.walk(og, seeds, 40, bias=bias)
+    assert np.array_equal(W, Wo) and steps == so
+
+
+def test_mdrw_matches_oracle(C, O, g14):
+    g, og, *_ = g14
+    S, FS, D = 64, 16, 50
+    seeds = g.seeds(S * FS, request=9)
+    so, do, oo = O.mdrw(og, seeds, S, FS, D)
+    src = np.zeros(S * D, np.int32)
+    dst = np.zeros(S * D, np.int32)
+    off = np.zeros(S + 1, np.int64)
+    st = C.Stats()
+    rc = C.lib().csaw_mdrw(g.h, 42, 0, S, FS, D, C._p(seeds), C._p(src), C._p(dst), C._p(off), C.ctypes.byref(st))
+    assert rc == 0
+    n = off[-1]
+    assert np.array_equal(off, oo) and np.array_equal(src[:n], so) and np.array_equal(dst[:n], do)
