@@ -55,7 +55,7 @@ class Tuning(ctypes.Structure):
 
 class PlanOpts(ctypes.Structure):
     _fields_ = [("mode", i32), ("sched", i32), ("variant", i32), ("use_graph", i32), ("timing", i32),
-                ("log_chunks", i32), ("explore_key", u64), ("mlp", P), ("tuning", P)]
+                ("log_chunks", i32), ("explore_key", u64), ("mlp", P), ("tuning", P), ("cdf_cache", i32)]
 
 
 if not os.path.exists(LIB_PATH):
@@ -87,14 +87,15 @@ _sig("csaw_edges_free", [P], None)
 _sig("csaw_random_walk", [P, i32, u64, u32, P, i64, i32, P, P])
 _sig("csaw_walk_plan_create", [P, i32, u64, u32, P, i64, i32, P])
 _sig("csaw_walk_plan_run", [P, P, ctypes.c_int])
-_sig("csaw_walk_plan_result", [P, P, P, P])
+_sig("csaw_walk_plan_result", [P, P, P, P, P])
 _sig("csaw_walk_plan_destroy", [P])
 _sig("csaw_plan_create", [P, P, i32, P, P])
 _sig("csaw_plan_run", [P, P, ctypes.c_int])
 _sig("csaw_plan_stats", [P, P])
 _sig("csaw_plan_request_info", [P, i32, P])
 _sig("csaw_plan_copy", [P, i32, P, P, P, P])
-_sig("csaw_plan_kernel_times", [P, ctypes.c_char_p, ctypes.c_size_t, P, P, i32, P])
+_sig("csaw_plan_kernel_times", [P, ctypes.c_char_p, ctypes.c_size_t, P, P, P, i32, P])
+_sig("csaw_plan_digest", [P, i32, P])
 _sig("csaw_plan_reset_times", [P])
 _sig("csaw_plan_destroy", [P])
 _sig("csaw_plan_chunk_log", [P, P, P, P, P, i64, P])
@@ -231,8 +232,10 @@ class WalkPlan:
 
     def result(self, walks=True):
         W = np.zeros((len(self.seeds), self.len + 1), np.int32) if walks else None
-        steps, ms = i64(), ctypes.c_double()
-        _check(_L.csaw_walk_plan_result(self.h, _p(W), ctypes.byref(steps), ctypes.byref(ms)), "walk_plan_result")
+        steps, ms, byt = i64(), ctypes.c_double(), ctypes.c_double()
+        _check(_L.csaw_walk_plan_result(self.h, _p(W), ctypes.byref(steps), ctypes.byref(ms), ctypes.byref(byt)),
+               "walk_plan_result")
+        self.algo_bytes = byt.value
         return W, steps.value, ms.value
 
     def close(self):
@@ -279,7 +282,7 @@ def request_mix(n=256, lo=64, hi=65536, mix_key=42):
 
 class Plan:
     def __init__(self, g, specs, mode=MODE_BATCHED, sched=SCHED_HEURISTIC, variant=0, use_graph=False, timing=False,
-                 log_chunks=False, explore_key=0, mlp=None, tuning=None):
+                 log_chunks=False, explore_key=0, mlp=None, tuning=None, cdf_cache=False):
         self.g = g
         self.specs = specs
         self.reqs, self._keep = make_requests(specs)
@@ -287,7 +290,8 @@ class Plan:
         opts = PlanOpts(mode=mode, sched=sched, variant=variant, use_graph=1 if use_graph else 0,
                         timing=1 if timing else 0, log_chunks=1 if log_chunks else 0, explore_key=explore_key,
                         mlp=mlp.h if mlp is not None else None,
-                        tuning=ctypes.cast(ctypes.byref(tuning), P) if tuning is not None else None)
+                        tuning=ctypes.cast(ctypes.byref(tuning), P) if tuning is not None else None,
+                        cdf_cache=1 if cdf_cache else 0)
         self.h = P()
         _check(_L.csaw_plan_create(g.h, self.reqs, len(specs), ctypes.byref(opts), ctypes.byref(self.h)),
                "plan_create")
@@ -316,15 +320,24 @@ class Plan:
         _check(_L.csaw_plan_copy(self.h, r, _p(src), _p(dst), _p(smp), _p(ho)), "plan_copy")
         return {"src": src[:n], "dst": dst[:n], "smp": smp[:n], "hop_off": ho[:D * (S + 1)].reshape(D, S + 1)}
 
-    def kernel_times(self):
+    def kernel_times(self, with_bytes=False):
+        """{kind: (avg ms per launch, launches[, algorithmic bytes per launch])}"""
         names = ctypes.create_string_buffer(1024)
         avg = np.zeros(32, np.float64)
         cnt = np.zeros(32, np.int64)
+        byt = np.zeros(32, np.float64)
         nk = i32()
-        _check(_L.csaw_plan_kernel_times(self.h, names, 1024, _p(avg), _p(cnt), 32, ctypes.byref(nk)),
+        _check(_L.csaw_plan_kernel_times(self.h, names, 1024, _p(avg), _p(cnt), _p(byt), 32, ctypes.byref(nk)),
                "plan_kernel_times")
         nm = names.raw.split(b"\0")
+        if with_bytes:
+            return {nm[i].decode(): (float(avg[i]), int(cnt[i]), float(byt[i])) for i in range(nk.value)}
         return {nm[i].decode(): (float(avg[i]), int(cnt[i])) for i in range(nk.value)}
+
+    def digest(self, r):
+        d = u64()
+        _check(_L.csaw_plan_digest(self.h, r, ctypes.byref(d)), "plan_digest")
+        return d.value
 
     def reset_times(self):
         _check(_L.csaw_plan_reset_times(self.h), "plan_reset_times")

@@ -63,6 +63,7 @@ struct GraphDev {
   const int32_t* hub_vid;       // sorted ids of vertices with degree > kHubMin
   const int64_t* hub_pool;      // pool offset per hub (n_hubs + 1)
   const uint64_t* hub_cdf[2];   // [DEGREE, USER]: inclusive uint64 CDFs of hubs
+  const uint64_t* int_cdf[2];   // full per-entry CDF cache (null when not used)
   int32_t n_hubs;
   int64_t V, E;
 };
@@ -106,6 +107,7 @@ struct HopDev {
   int32_t* smp;
   int64_t* eo;          // per-sample edge offsets of this hop
   uint32_t* flags;      // bit 0: chunk overflow
+  unsigned long long* bytes;  // algorithmic bytes per kernel kind (null: not counted)
 };
 
 __host__ __device__ __forceinline__ int ceil_log2_u32(uint32_t d) {
@@ -134,6 +136,23 @@ __host__ __device__ __forceinline__ int entry_bucket(int64_t d, int32_t F, int32
   int kind = bias_kind(bias);
   int l = ceil_log2_u32((uint32_t)(d > 0x7fffffff ? 0x7fffffff : d));
   return 1 + 32 * kind + l;
+}
+
+// Algorithmic bytes of one expansion (each datum counted once per use, no
+// cache-line amplification): frontier entry 8 + offset 8 + degree 4 = 20;
+// per emitted edge the picked id 4 + the edge written (src, dst, sample) 12.
+// Integer biases built on the fly read adj + degree of every neighbour (8d);
+// a cached CDF costs its total plus ceil(log2 d) probes per draw.
+//   copy-all:  20 + 16d          uniform: 20 + 16k
+//   int build: 20 + 8d + 16k     int cached: 28 + k(8l + 16)
+//   fp32:      24 + k(4l + 16)   (l = ceil(log2 d))
+__host__ __device__ __forceinline__ uint64_t expansion_bytes(int kind, bool built, int64_t n, int64_t sum_d,
+                                                             int64_t sum_k, int l, bool copy) {
+  if (copy) return 20 * n + 16 * sum_d;
+  if (kind == kUniform) return 20 * n + 16 * sum_k;
+  if (kind == kF32) return 24 * n + sum_k * (4 * l + 16);
+  if (built) return 20 * n + 8 * sum_d + 16 * sum_k;
+  return 28 * n + sum_k * (8 * l + 16);
 }
 
 // ------------------------------------------------------- wave primitives

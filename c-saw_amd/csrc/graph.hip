@@ -329,11 +329,68 @@ int graph_prepare_bias(csaw_graph* g, int32_t bias) {
   return CSAW_OK;
 }
 
+// Full integer CDF cache (the reference's probability cache Cp,
+// functions.cuh:190-208 and 329-341, there compiled in but disabled by
+// cache=0 at main.cu:26): one inclusive uint64 CDF per CSR entry, so a draw
+// costs a binary search instead of a pass over the neighbour list.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_int_cdf_small(GraphDev G, csaw_graph_view hv, uint64_t* __restrict__ C) {
+  const int64_t nw = (int64_t)gridDim.x * 4;
+  const int lane = lane_id();
+  for (int64_t v = blockIdx.x * 4 + threadIdx.x / kWave; v < G.V; v += nw) {
+    const int32_t d = G.deg[v];
+    if (d == 0 || d > kHubMin) continue;
+    const int64_t o = G.off[v];
+    uint64_t acc = 0;
+    for (int i0 = 0; i0 < d; i0 += kWave) {
+      const int i = i0 + lane;
+      uint64_t w = 0;
+      if (i < d) {
+        const int32_t u = G.adj[o + i];
+        w = KIND == 0 ? (uint64_t)G.deg[u] : (uint64_t)EdgeBias(u, &hv);
+      }
+      const uint64_t s = wave_incl_scan_u64(w);
+      if (i < d) C[o + i] = acc + s;
+      acc += readlane_u64(s, kWave - 1);
+    }
+  }
+}
+
+__global__ void k_int_cdf_hubs(GraphDev G, const uint64_t* __restrict__ pool, uint64_t* __restrict__ C) {
+  for (int32_t h = blockIdx.x; h < G.n_hubs; h += gridDim.x) {
+    const int32_t v = G.hub_vid[h];
+    const int64_t o = G.off[v], d = G.off[v + 1] - o, p = G.hub_pool[h];
+    for (int64_t i = threadIdx.x; i < d; i += blockDim.x) C[o + i] = pool[p + i];
+  }
+}
+
+int graph_prepare_int_cache(csaw_graph* g, int32_t bias) {
+  if (bias != CSAW_BIAS_DEGREE && bias != CSAW_BIAS_USER) return CSAW_OK;
+  const int kind = bias == CSAW_BIAS_DEGREE ? 0 : 1;
+  if (g->int_cdf[kind]) return CSAW_OK;
+  CSAW_TRY(graph_prepare_bias(g, bias));
+  CSAW_HIP(hipMalloc(&g->int_cdf[kind], sizeof(uint64_t) * (g->E > 0 ? g->E : 1)));
+  GraphDev G = g->view();
+  csaw_graph_view hv = g->hook_view();
+  if (g->V > 0) {
+    const int grid = grid_for((g->V + 3) / 4, 1, g->num_cu);
+    if (kind == 0) k_int_cdf_small<0><<<grid, 256, 0, g->stream>>>(G, hv, g->int_cdf[kind]);
+    else k_int_cdf_small<1><<<grid, 256, 0, g->stream>>>(G, hv, g->int_cdf[kind]);
+    CSAW_LAUNCH_CHECK();
+  }
+  if (g->n_hubs > 0) {
+    k_int_cdf_hubs<<<std::min(g->n_hubs, g->num_cu * 8), 256, 0, g->stream>>>(G, g->hub_cdf[kind], g->int_cdf[kind]);
+    CSAW_LAUNCH_CHECK();
+  }
+  CSAW_HIP(hipStreamSynchronize(g->stream));
+  return CSAW_OK;
+}
+
 }  // namespace csaw
 
 using namespace csaw;
 
-csaw::GraphDev csaw_graph::view() const {
+csaw::GraphDev csaw_graph::view(bool use_cache) const {
   GraphDev G;
   G.off = off;
   G.adj = adj;
@@ -344,6 +401,8 @@ csaw::GraphDev csaw_graph::view() const {
   G.hub_pool = hub_pool;
   G.hub_cdf[0] = hub_cdf[0];
   G.hub_cdf[1] = hub_cdf[1];
+  G.int_cdf[0] = use_cache ? int_cdf[0] : nullptr;
+  G.int_cdf[1] = use_cache ? int_cdf[1] : nullptr;
   G.n_hubs = n_hubs < 0 ? 0 : n_hubs;
   G.V = V;
   G.E = E;
@@ -372,6 +431,8 @@ static void graph_free(csaw_graph* g) {
   (void)hipFree(g->hub_pool);
   (void)hipFree(g->hub_cdf[0]);
   (void)hipFree(g->hub_cdf[1]);
+  (void)hipFree(g->int_cdf[0]);
+  (void)hipFree(g->int_cdf[1]);
   if (g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
 }

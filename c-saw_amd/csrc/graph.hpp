@@ -17,12 +17,13 @@ struct csaw_graph {
   int32_t* hub_vid = nullptr;
   int64_t* hub_pool = nullptr;  // n_hubs+1 pool offsets
   uint64_t* hub_cdf[2] = {nullptr, nullptr};  // [DEGREE, USER]
+  uint64_t* int_cdf[2] = {nullptr, nullptr};  // full per-entry CDF cache [DEGREE, USER]
   int64_t hub_entries = 0;
   hipStream_t stream = nullptr;
   int device = 0;
   int num_cu = 0;
 
-  csaw::GraphDev view() const;
+  csaw::GraphDev view(bool use_cache = false) const;
   csaw_graph_view hook_view() const;
 };
 
@@ -30,6 +31,7 @@ namespace csaw {
 // Builds lazily what a bias kind needs: FP32 CDF (CSAW_BIAS_WEIGHT) or the
 // integer hub cache (CSAW_BIAS_DEGREE / CSAW_BIAS_USER).
 int graph_prepare_bias(csaw_graph* g, int32_t bias);
+int graph_prepare_int_cache(csaw_graph* g, int32_t bias);
 int graph_alloc_common(csaw_graph* g);
 int device_init(int* num_cu);
 }  // namespace csaw

@@ -104,6 +104,7 @@ static void plan_free(csaw_plan* p) {
   (void)hipFree(p->chosen);
   (void)hipFree(p->log);
   (void)hipFree(p->d_mlp);
+  (void)hipFree(p->bytes_acc);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
@@ -225,11 +226,12 @@ static HopDev make_hop(csaw_plan* p, Unit& u, int h) {
   H.smp = u.d_smp[h];
   H.eo = u.d_eo[h];
   H.flags = p->counters + kCtrFlags;
+  H.bytes = p->opts.timing ? p->bytes_acc : nullptr;
   return H;
 }
 
 static int run_unit(csaw_plan* p, Unit& u, hipStream_t st, TimingHook* th) {
-  const GraphDev G = p->g->view();
+  const GraphDev G = p->g->view(p->opts.cdf_cache != 0);
   const csaw_graph_view hv = p->g->hook_view();
   LaunchCfg L = p->L;
   L.stream = st;
@@ -256,6 +258,7 @@ static int run_unit(csaw_plan* p, Unit& u, hipStream_t st, TimingHook* th) {
       A.queues = p->queues;
       A.queue_cap = p->queue_cap;
       A.chosen = p->chosen;
+      A.int_cache = p->opts.cdf_cache != 0;
       A.mlp.w = p->d_mlp;
       A.mlp_w = p->d_mlp;
       if (th) th->begin(kKernChoose, st);
@@ -325,6 +328,7 @@ extern "C" int csaw_plan_create(csaw_graph* g, const csaw_request* reqs, int32_t
   auto body = [&]() -> int {
     for (int i = 0; i < n_req; ++i) {
       CSAW_TRY(graph_prepare_bias(g, reqs[i].bias));
+      if (o.cdf_cache) CSAW_TRY(graph_prepare_int_cache(g, reqs[i].bias));
       p->reqs.push_back(reqs[i]);
       p->seeds.emplace_back(reqs[i].seeds, reqs[i].seeds + reqs[i].samples);
       p->reqs.back().seeds = nullptr;
@@ -355,6 +359,8 @@ extern "C" int csaw_plan_create(csaw_graph* g, const csaw_request* reqs, int32_t
     CSAW_HIP(hipMalloc(&p->counters, sizeof(uint32_t) * kCtrCount));
     CSAW_HIP(hipMemset(p->counters, 0, sizeof(uint32_t) * kCtrCount));
     CSAW_HIP(hipMalloc(&p->totals, sizeof(int64_t) * depth));
+    CSAW_HIP(hipMalloc(&p->bytes_acc, sizeof(unsigned long long) * kNumKernKinds));
+    CSAW_HIP(hipMemset(p->bytes_acc, 0, sizeof(unsigned long long) * kNumKernKinds));
     if (o.mode == CSAW_MODE_BATCHED) {
       CSAW_HIP(hipMalloc(&p->order, sizeof(int32_t) * nblk * kBlockEntries));
       p->chunk_cap = (uint32_t)std::min<int64_t>(nblk * kChunksPerBlock, 0xFFFFFFF0LL);
@@ -396,6 +402,11 @@ extern "C" int csaw_plan_run(csaw_plan* p, void* stream, int sync) {
   CSAW_HIP(hipEventCreate(&e1));
   CSAW_HIP(hipEventRecord(e0, st));
   TimingHook* th = p->timer.enabled ? &p->timer : nullptr;
+  if (th) {
+    LaunchCfg L = p->L;
+    L.stream = st;
+    CSAW_TRY(launch_fill(L, (uint32_t*)p->bytes_acc, 0u, 2 * kNumKernKinds));
+  }
   const bool use_graph = p->opts.use_graph && p->opts.mode == CSAW_MODE_BATCHED && !th && !p->opts.log_chunks;
   if (use_graph) {
     if (!p->gexec || p->graph_stream != st) {
@@ -430,7 +441,12 @@ extern "C" int csaw_plan_run(csaw_plan* p, void* stream, int sync) {
     float ms = 0.f;
     CSAW_HIP(hipEventElapsedTime(&ms, e0, e1));
     p->last_ms = ms;
-    if (th) p->timer.collect();
+    if (th) {
+      p->timer.collect();
+      unsigned long long b[kNumKernKinds];
+      CSAW_HIP(hipMemcpy(b, p->bytes_acc, sizeof(b), hipMemcpyDeviceToHost));
+      for (int k = 0; k < kNumKernKinds; ++k) p->timer.total_bytes[k] += (double)b[k];
+    }
     uint32_t flags = 0;
     CSAW_HIP(hipMemcpy(&flags, p->counters + kCtrFlags, sizeof(uint32_t), hipMemcpyDeviceToHost));
     if (flags & 1u) {
@@ -501,6 +517,31 @@ extern "C" int csaw_plan_copy(const csaw_plan* p, int32_t req, int32_t* src, int
   return CSAW_OK;
 }
 
+extern "C" int csaw_plan_digest(const csaw_plan* p, int32_t req, uint64_t* digest) {
+  if (!p || !digest) return CSAW_ERR_INVALID_ARG;
+  std::vector<int64_t> lo, hi;
+  CSAW_TRY(request_bounds(p, req, lo, hi));
+  const auto [ui, k] = p->req_unit[req];
+  const Unit& u = p->units[ui];
+  unsigned long long* d;
+  CSAW_HIP(hipMalloc(&d, sizeof(unsigned long long)));
+  CSAW_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), p->stream));
+  int64_t at = 0;
+  int st = CSAW_OK;
+  for (size_t h = 0; h < lo.size() && st == CSAW_OK; ++h) {
+    st = launch_digest(p->stream, p->g->num_cu, u.d_src[h], u.d_dst[h], u.d_smp[h], lo[h], hi[h], at, u.gbase[k], d);
+    at += hi[h] - lo[h];
+  }
+  unsigned long long v = 0;
+  if (st == CSAW_OK) {
+    CSAW_HIP(hipMemcpyAsync(&v, d, sizeof(v), hipMemcpyDeviceToHost, p->stream));
+    CSAW_HIP(hipStreamSynchronize(p->stream));
+  }
+  (void)hipFree(d);
+  *digest = (uint64_t)v ^ ((uint64_t)at * 0x100000001b3ull);
+  return st;
+}
+
 extern "C" int csaw_plan_stats(const csaw_plan* p, csaw_stats* s) {
   if (!p || !s) return CSAW_ERR_INVALID_ARG;
   memset(s, 0, sizeof(*s));
@@ -522,7 +563,7 @@ extern "C" int csaw_plan_stats(const csaw_plan* p, csaw_stats* s) {
 }
 
 extern "C" int csaw_plan_kernel_times(const csaw_plan* p, char* names, size_t names_len, double* avg_ms,
-                                      int64_t* launches, int32_t max_kinds, int32_t* n_kinds) {
+                                      int64_t* launches, double* avg_bytes, int32_t max_kinds, int32_t* n_kinds) {
   if (!p) return CSAW_ERR_INVALID_ARG;
   int32_t n = 0;
   size_t pos = 0;
@@ -531,6 +572,7 @@ extern "C" int csaw_plan_kernel_times(const csaw_plan* p, char* names, size_t na
     if (n < max_kinds) {
       if (avg_ms) avg_ms[n] = p->timer.total_ms[k] / (double)p->timer.launches[k];
       if (launches) launches[n] = p->timer.launches[k];
+      if (avg_bytes) avg_bytes[n] = p->timer.total_bytes[k] / (double)p->timer.launches[k];
       if (names) {
         size_t l = strlen(kKernNames[k]) + 1;
         if (pos + l <= names_len) {
@@ -550,6 +592,7 @@ extern "C" int csaw_plan_reset_times(csaw_plan* p) {
   for (int k = 0; k < kNumKernKinds; ++k) {
     p->timer.total_ms[k] = 0;
     p->timer.launches[k] = 0;
+    p->timer.total_bytes[k] = 0;
   }
   return CSAW_OK;
 }

@@ -27,6 +27,7 @@ struct EventTimer : TimingHook {
   size_t used = 0;
   double total_ms[kNumKernKinds] = {};
   int64_t launches[kNumKernKinds] = {};
+  double total_bytes[kNumKernKinds] = {};
   void begin(int which, hipStream_t s) override;
   void end(int which, hipStream_t s) override;
   void collect();
@@ -82,6 +83,7 @@ struct csaw_plan {
   uint8_t* chosen = nullptr;
   csaw::ChunkLog* log = nullptr;
   uint16_t* d_mlp = nullptr;
+  unsigned long long* bytes_acc = nullptr;  // algorithmic bytes per kernel kind (timing runs)
   csaw::LaunchCfg L;
   hipStream_t stream = nullptr;
   bool own_stream = false;

@@ -49,6 +49,7 @@ __global__ __launch_bounds__(256) void k_count(GraphDev G, const ReqDev* __restr
     int bk[4];
     uint32_t slot[4];
     uint32_t tsum = 0;
+    unsigned long long tbytes = 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int64_t e = base + tid * 4 + i;
@@ -60,6 +61,13 @@ __global__ __launch_bounds__(256) void k_count(GraphDev G, const ReqDev* __restr
         const int32_t F = r.fanout[H.hop];
         const int32_t d = G.deg[v];
         k[i] = (uint32_t)entry_k(d, F, r.dedup);
+        if (!BIN && H.bytes) {
+          const int kind = bias_kind(r.bias);
+          const bool built = kind == kInt && !G.int_cdf[r.bias == CSAW_BIAS_USER ? 1 : 0] && d <= kWaveIntCap;
+          tbytes += k[i] == 0 ? 20ull
+                              : expansion_bytes(kind, built, 1, d, k[i], ceil_log2_u32((uint32_t)d),
+                                                r.dedup && d <= F);
+        }
         if (BIN && k[i] > 0) {
           bk[i] = entry_bucket(d, F, r.dedup, r.bias);
           slot[i] = atomicAdd(&hist[bk[i]], 1u);
@@ -70,6 +78,10 @@ __global__ __launch_bounds__(256) void k_count(GraphDev G, const ReqDev* __restr
         }
       }
       tsum += k[i];
+    }
+    if (!BIN && H.bytes) {
+      for (int off = 32; off >= 1; off >>= 1) tbytes += __shfl_xor(tbytes, off, 64);
+      if (lane == 0 && tbytes) atomicAdd(&H.bytes[kKernPlain], tbytes);
     }
     // block exclusive scan of per-thread sums
     const uint32_t incl = wave_incl_scan_u32(tsum);
@@ -198,6 +210,41 @@ __global__ void k_offsets(HopDev H, const int64_t* __restrict__ total) {
   }
 }
 
+// ------------------------------------------------------------ digest
+// Order-sensitive digest of an edge range: sum over positions of a 64-bit
+// mix of (position, src, dst, sample); the sum makes it independent of how
+// the reduction is scheduled.
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return x;
+}
+
+__global__ void k_digest(const int32_t* __restrict__ src, const int32_t* __restrict__ dst,
+                         const int32_t* __restrict__ smp, int64_t lo, int64_t hi, int64_t pos0, int64_t gbase,
+                         unsigned long long* __restrict__ out) {
+  uint64_t acc = 0;
+  for (int64_t i = lo + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < hi; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t a = (uint64_t)(uint32_t)src[i] | ((uint64_t)(uint32_t)dst[i] << 32);
+    const uint64_t b = (uint64_t)(pos0 + i - lo) * 0x9E3779B97F4A7C15ull ^ (uint64_t)(uint32_t)(smp[i] - gbase);
+    acc += mix64(a ^ mix64(b));
+  }
+  for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if (lane_id() == 0 && acc) atomicAdd(out, (unsigned long long)acc);
+}
+
+int launch_digest(hipStream_t st, int num_cu, const int32_t* src, const int32_t* dst, const int32_t* smp, int64_t lo,
+                  int64_t hi, int64_t pos0, int64_t gbase, unsigned long long* out) {
+  if (hi <= lo) return CSAW_OK;
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((hi - lo + 255) / 256, (int64_t)num_cu * 8));
+  k_digest<<<grid, 256, 0, st>>>(src, dst, smp, lo, hi, pos0, gbase, out);
+  CSAW_LAUNCH_CHECK();
+  return CSAW_OK;
+}
+
 // --------------------------------------------------------- fills / resets
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
@@ -220,12 +267,24 @@ __global__ __launch_bounds__(256) void k_expand_plain(GraphDev G, csaw_graph_vie
 }
 
 // ------------------------------------------------------------- choose
-__device__ __forceinline__ bool variant_valid(int var, int kind, const Chunk& c) {
-  if (var == CSAW_VAR_THREAD) return c.max_f <= kThreadFanCap && (kind != kInt || c.max_d <= (uint32_t)kThreadIntCap);
+// THREAD: fanout <= 32 and, for integer biases without the CDF cache,
+// degree <= 32 (per-lane LDS weights).  BLOCK only pays for integer biases
+// (256 lanes build the CDF); for uniform and FP32 it would idle 3 waves.
+__device__ __forceinline__ bool variant_valid(int var, int kind, const Chunk& c, bool int_cache) {
+  if (var == CSAW_VAR_THREAD)
+    return c.max_f <= kThreadFanCap && (kind != kInt || int_cache || c.max_d <= (uint32_t)kThreadIntCap);
+  if (var == CSAW_VAR_BLOCK_BITMAP || var == CSAW_VAR_BLOCK_HASH) return kind == kInt;
   return true;
 }
 
-__device__ int heuristic_variant(int kind, const Chunk& c);
+// The fixed heuristic (C-SAW degree thresholds SML_MID=32, MID_LRG=1024,
+// util.h:19-20, with the reference's default bitmap dedup, main.cu:26).
+__device__ int heuristic_variant(int kind, const Chunk& c, bool cache) {
+  const uint32_t d = c.max_d;
+  if (d < 32 && variant_valid(CSAW_VAR_THREAD, kind, c, cache)) return CSAW_VAR_THREAD;
+  if (d < 1024 || !variant_valid(CSAW_VAR_BLOCK_BITMAP, kind, c, cache)) return CSAW_VAR_WAVE_BITMAP;
+  return CSAW_VAR_BLOCK_BITMAP;
+}
 
 // Learned choice: argmin of the predicted log2 time over the valid variants.
 __device__ int mlp_choose(const ChooseArgs& A, const Chunk& c, int kind, const HopDev& H) {
@@ -235,20 +294,11 @@ __device__ int mlp_choose(const ChooseArgs& A, const Chunk& c, int kind, const H
   int best = -1;
   float bv = 3.0e38f;
   for (int v = 0; v < kMlpOut; ++v)
-    if (variant_valid(v, kind, c) && o[v] < bv) {
+    if (variant_valid(v, kind, c, A.int_cache) && o[v] < bv) {
       bv = o[v];
       best = v;
     }
-  return best >= 0 ? best : heuristic_variant(kind, c);
-}
-
-// The fixed heuristic (C-SAW degree thresholds SML_MID=32, MID_LRG=1024,
-// util.h:19-20, with the reference's default bitmap dedup, main.cu:26).
-__device__ int heuristic_variant(int kind, const Chunk& c) {
-  const uint32_t d = c.max_d;
-  if (d < 32 && variant_valid(CSAW_VAR_THREAD, kind, c)) return CSAW_VAR_THREAD;
-  if (d < 1024) return CSAW_VAR_WAVE_BITMAP;
-  return CSAW_VAR_BLOCK_BITMAP;
+  return best >= 0 ? best : heuristic_variant(kind, c, A.int_cache);
 }
 
 __global__ void k_choose(ChooseArgs A, HopDev H) {
@@ -260,22 +310,31 @@ __global__ void k_choose(ChooseArgs A, HopDev H) {
       const Chunk c = H.chunks[ci];
       if (c.bucket == kBucketCopy) {
         q = kQueueCopy;
+        if (H.bytes) atomicAdd(&H.bytes[kKernCopy], (unsigned long long)expansion_bytes(0, false, c.len, (int64_t)c.sum_d, 0, 0, true));
       } else {
         const int kind = (c.bucket - 1) >> 5;
-        int var = heuristic_variant(kind, c);
+        int var = heuristic_variant(kind, c, A.int_cache);
         if (A.sched == CSAW_SCHED_FORCED) {
-          if (variant_valid(A.variant, kind, c)) var = A.variant;
+          if (variant_valid(A.variant, kind, c, A.int_cache)) var = A.variant;
         } else if (A.sched == CSAW_SCHED_RANDOM) {
           const U4 o = philox4x32_10(c.key, (uint32_t)c.hop, 0u, kDomExplore << 28, (uint32_t)A.explore_key,
                                      (uint32_t)(A.explore_key >> 32));
           int cand = (int)(o.x % CSAW_NUM_VARIANTS);
-          for (int t = 0; t < CSAW_NUM_VARIANTS && !variant_valid(cand, kind, c); ++t) cand = (cand + 1) % CSAW_NUM_VARIANTS;
+          for (int t = 0; t < CSAW_NUM_VARIANTS && !variant_valid(cand, kind, c, A.int_cache); ++t)
+            cand = (cand + 1) % CSAW_NUM_VARIANTS;
           var = cand;
         } else if (A.sched == CSAW_SCHED_MLP && A.mlp_w) {
           var = mlp_choose(A, c, kind, H);
         }
         q = var * 3 + kind;
         if (A.chosen) A.chosen[ci] = (uint8_t)var;
+        if (H.bytes) {
+          const int l = (c.bucket - 1) & 31;
+          const int kk = var == CSAW_VAR_THREAD ? kKernThread : (var <= CSAW_VAR_WAVE_HASH ? kKernWave : kKernBlock);
+          const int cap_l = var == CSAW_VAR_THREAD ? 5 : (var <= CSAW_VAR_WAVE_HASH ? 10 : 13);
+          const bool built = kind == kInt && !A.int_cache && l <= cap_l;
+          atomicAdd(&H.bytes[kk], (unsigned long long)expansion_bytes(kind, built, c.len, (int64_t)c.sum_d, c.sum_f, l, false));
+        }
       }
     }
     // wave-aggregated append to queue q

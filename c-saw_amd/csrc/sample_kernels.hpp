@@ -63,6 +63,7 @@ struct ChooseArgs {
   uint32_t* queues;
   int64_t queue_cap;
   uint8_t* chosen;   // per chunk variant (may be null)
+  int32_t int_cache; // integer CDF cache available (THREAD valid for any degree)
   MlpDev mlp;
   const void* mlp_w; // non-null when a model is loaded
 };
@@ -80,6 +81,8 @@ int launch_count(const LaunchCfg& L, const GraphDev& G, const ReqDev* R, const i
                  bool bin, int64_t max_entries);
 int launch_scans(const LaunchCfg& L, const HopDev& H, int64_t* total, int64_t max_entries);
 int launch_fill(const LaunchCfg& L, uint32_t* p, uint32_t v, int64_t n);
+int launch_digest(hipStream_t st, int num_cu, const int32_t* src, const int32_t* dst, const int32_t* smp, int64_t lo,
+                  int64_t hi, int64_t pos0, int64_t gbase, unsigned long long* out);
 int launch_expand_plain(const LaunchCfg& L, int kind, const GraphDev& G, const csaw_graph_view& hv, const ReqDev* R,
                         const int32_t* gs2req, const HopDev& H, int64_t max_entries);
 int launch_choose(const LaunchCfg& L, const ChooseArgs& A, const HopDev& H, int64_t max_chunks);

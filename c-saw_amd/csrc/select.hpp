@@ -92,9 +92,14 @@ __device__ __forceinline__ bool picker_global(const GraphDev& G, const Entry& en
     if (pk.Tf > 0.0f) pk.mode = 3;
     return true;
   }
-  if (en.d <= cap) return false;
-  const int32_t h = hub_find(G.hub_vid, G.n_hubs, en.v);
-  pk.C = G.hub_cdf[en.bias == CSAW_BIAS_USER ? 1 : 0] + G.hub_pool[h];
+  const int ck = en.bias == CSAW_BIAS_USER ? 1 : 0;
+  if (G.int_cdf[ck]) {
+    pk.C = G.int_cdf[ck] + en.o;
+  } else {
+    if (en.d <= cap) return false;
+    const int32_t h = hub_find(G.hub_vid, G.n_hubs, en.v);
+    pk.C = G.hub_cdf[ck] + G.hub_pool[h];
+  }
   pk.T = pk.C[en.d - 1];
   if (pk.T > 0) pk.mode = 2;
   return true;
@@ -245,10 +250,15 @@ __device__ void expand_entry_thread(const GraphDev& G, const csaw_graph_view& hv
   uint64_t T = 0;
   float Tf = 0.f;
   const float* Cf = nullptr;
+  const uint64_t* Cg = nullptr;
   if (KIND == kF32) {
     Cf = G.cdf + en.o;
     Tf = Cf[d - 1];
     if (Tf > 0.0f) mode = 3;
+  } else if (KIND == kInt && G.int_cdf[en.bias == CSAW_BIAS_USER ? 1 : 0]) {
+    Cg = G.int_cdf[en.bias == CSAW_BIAS_USER ? 1 : 0] + en.o;
+    T = Cg[d - 1];
+    if (T > 0) mode = 2;
   } else if (KIND == kInt) {
     for (int32_t i = 0; i < d; ++i) {
       const uint32_t w = int_bias(G, hv, en.bias, G.adj[en.o + i]);
@@ -262,6 +272,7 @@ __device__ void expand_entry_thread(const GraphDev& G, const csaw_graph_view& hv
     if (mode == 0) return (int32_t)mulhi64(t, (uint64_t)d);
     if (mode == 3) return (int32_t)upper_bound_dev<float>(Cf, d, __fmul_rn(philox_u24(o), Tf));
     const uint64_t x = mulhi64(t, T);
+    if (mode == 2) return (int32_t)upper_bound_dev<uint64_t>(Cg, d, x);
     uint64_t acc = 0;
     int32_t i = 0;
     for (; i < d - 1; ++i) {

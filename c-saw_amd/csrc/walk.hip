@@ -57,7 +57,7 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk(GraphDev G, csaw_graph_vi
                                                        unsigned long long* __restrict__ steps_out) {
   __shared__ int32_t tile[kWalkTile][kWalkThreads + 1];
   const uint32_t c3 = (kDomWalk << 28) | request;
-  unsigned long long moved = 0;
+  unsigned long long moved = 0, bytes = 0;
   for (int64_t w0 = (int64_t)blockIdx.x * kWalkThreads; w0 < n; w0 += (int64_t)gridDim.x * kWalkThreads) {
     const int64_t w = w0 + threadIdx.x;
     const bool act = w < n;
@@ -77,6 +77,12 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk(GraphDev G, csaw_graph_vi
             const U4 r = philox4x32_10((uint32_t)t, (uint32_t)w, (uint32_t)(w >> 32), c3, k0, k1);
             v = G.adj[o + walk_pick<KIND>(G, hv, bias, v, o, d, r)];
             ++moved;
+            // algorithmic bytes of the step: offset 8 + degree 4 + picked id 4 +
+            // walk entry 4, plus the CDF total and ceil(log2 d) probes (FP32 /
+            // cached integer) or two passes over adj + degree (integer, built)
+            const int l = ceil_log2_u32((uint32_t)d);
+            bytes += 20 + (KIND == kF32 ? 4 + 4 * l
+                                        : (KIND == kInt ? (d > kHubMin ? 8 + 8 * l : 16 * (unsigned long long)d) : 0));
           }
         }
         tile[k][threadIdx.x] = v;
@@ -86,9 +92,15 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk(GraphDev G, csaw_graph_vi
     }
   }
   // wave-aggregated step count
-  unsigned long long s = moved;
-  for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
-  if (lane_id() == 0 && s) atomicAdd(steps_out, s);
+  unsigned long long s = moved, b = bytes;
+  for (int off = 32; off >= 1; off >>= 1) {
+    s += __shfl_xor(s, off, 64);
+    b += __shfl_xor(b, off, 64);
+  }
+  if (lane_id() == 0 && s) {
+    atomicAdd(steps_out, s);
+    atomicAdd(steps_out + 1, b);
+  }
 }
 
 int launch_walk(hipStream_t st, int num_cu, int per_cu, const GraphDev& G, const csaw_graph_view& hv, int32_t bias,
@@ -128,6 +140,7 @@ struct csaw_walk_plan {
   int per_cu = 8;
   double last_ms = 0.0;
   int64_t last_steps = 0;
+  double last_bytes = 0.0;
 };
 
 extern "C" int csaw_walk_plan_create(csaw_graph* g, int32_t bias, uint64_t seed, uint32_t request_id,
@@ -148,7 +161,7 @@ extern "C" int csaw_walk_plan_create(csaw_graph* g, int32_t bias, uint64_t seed,
   auto body = [&]() -> int {
     CSAW_HIP(hipMalloc(&p->d_seeds, sizeof(int32_t) * std::max<int64_t>(n_walkers, 1)));
     CSAW_HIP(hipMalloc(&p->d_W, sizeof(int32_t) * std::max<int64_t>(n_walkers * (len + 1), 1)));
-    CSAW_HIP(hipMalloc(&p->d_steps, sizeof(unsigned long long)));
+    CSAW_HIP(hipMalloc(&p->d_steps, 2 * sizeof(unsigned long long)));
     if (n_walkers > 0)
       CSAW_HIP(hipMemcpy(p->d_seeds, seeds, sizeof(int32_t) * n_walkers, hipMemcpyHostToDevice));
     CSAW_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
@@ -172,7 +185,7 @@ extern "C" int csaw_walk_plan_run(csaw_walk_plan* p, void* stream, int sync) {
   hipEvent_t e0, e1;
   CSAW_HIP(hipEventCreate(&e0));
   CSAW_HIP(hipEventCreate(&e1));
-  CSAW_HIP(hipMemsetAsync(p->d_steps, 0, sizeof(unsigned long long), st));
+  CSAW_HIP(hipMemsetAsync(p->d_steps, 0, 2 * sizeof(unsigned long long), st));
   CSAW_HIP(hipEventRecord(e0, st));
   CSAW_TRY(launch_walk(st, p->g->num_cu, p->per_cu, p->g->view(), p->g->hook_view(), p->bias, p->key, p->request,
                        p->d_seeds, p->n, p->len, p->d_W, p->d_steps));
@@ -182,21 +195,24 @@ extern "C" int csaw_walk_plan_run(csaw_walk_plan* p, void* stream, int sync) {
     float ms = 0.f;
     CSAW_HIP(hipEventElapsedTime(&ms, e0, e1));
     p->last_ms = ms;
-    unsigned long long s = 0;
-    CSAW_HIP(hipMemcpy(&s, p->d_steps, sizeof(s), hipMemcpyDeviceToHost));
-    p->last_steps = (int64_t)s;
+    unsigned long long s[2] = {0, 0};
+    CSAW_HIP(hipMemcpy(s, p->d_steps, sizeof(s), hipMemcpyDeviceToHost));
+    p->last_steps = (int64_t)s[0];
+    p->last_bytes = (double)s[1];
   }
   CSAW_HIP(hipEventDestroy(e0));
   CSAW_HIP(hipEventDestroy(e1));
   return CSAW_OK;
 }
 
-extern "C" int csaw_walk_plan_result(const csaw_walk_plan* p, int32_t* walks, int64_t* steps, double* kernel_ms) {
+extern "C" int csaw_walk_plan_result(const csaw_walk_plan* p, int32_t* walks, int64_t* steps, double* kernel_ms,
+                                     double* algo_bytes) {
   if (!p) return CSAW_ERR_INVALID_ARG;
   if (walks && p->n > 0)
     CSAW_HIP(hipMemcpy(walks, p->d_W, sizeof(int32_t) * p->n * (p->len + 1), hipMemcpyDeviceToHost));
   if (steps) *steps = p->last_steps;
   if (kernel_ms) *kernel_ms = p->last_ms;
+  if (algo_bytes) *algo_bytes = p->last_bytes;
   return CSAW_OK;
 }
 
@@ -215,7 +231,7 @@ extern "C" int csaw_random_walk(csaw_graph* g, int32_t bias, uint64_t seed, uint
   csaw_walk_plan* p = nullptr;
   CSAW_TRY(csaw_walk_plan_create(g, bias, seed, request_id, seeds, n_walkers, len, &p));
   int st = csaw_walk_plan_run(p, nullptr, 1);
-  if (st == CSAW_OK) st = csaw_walk_plan_result(p, walks, nullptr, nullptr);
+  if (st == CSAW_OK) st = csaw_walk_plan_result(p, walks, nullptr, nullptr, nullptr);
   if (st == CSAW_OK && stats) {
     memset(stats, 0, sizeof(*stats));
     stats->edges = p->last_steps;

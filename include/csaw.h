@@ -190,7 +190,8 @@ typedef struct csaw_walk_plan csaw_walk_plan;
 int csaw_walk_plan_create(csaw_graph *g, int32_t bias, uint64_t seed, uint32_t request_id,
                           const int32_t *seeds, int64_t n_walkers, int32_t len, csaw_walk_plan **out);
 int csaw_walk_plan_run(csaw_walk_plan *p, void *stream, int sync);
-int csaw_walk_plan_result(const csaw_walk_plan *p, int32_t *walks, int64_t *steps, double *kernel_ms);
+int csaw_walk_plan_result(const csaw_walk_plan *p, int32_t *walks, int64_t *steps, double *kernel_ms,
+                          double *algo_bytes);
 int csaw_walk_plan_destroy(csaw_walk_plan *p);
 
 /* ------------------------------------------------ plans (batched, reusable) */
@@ -215,6 +216,8 @@ typedef struct csaw_plan_opts {
   uint64_t explore_key; /* key of CSAW_SCHED_RANDOM                        */
   const struct csaw_mlp *mlp; /* cost model for CSAW_SCHED_MLP (or NULL)   */
   const struct csaw_tuning *tuning; /* launch shapes (or NULL: defaults)   */
+  int32_t cdf_cache; /* integer-bias draws use the per-graph CDF cache (the */
+                     /* reference's Cp, functions.cuh:329-341; off there)   */
 } csaw_plan_opts;
 
 typedef struct csaw_plan csaw_plan;
@@ -227,10 +230,15 @@ int csaw_plan_stats(const csaw_plan *p, csaw_stats *stats);
 int csaw_plan_request_info(const csaw_plan *p, int32_t req, int64_t *n_edges);
 int csaw_plan_copy(const csaw_plan *p, int32_t req, int32_t *src, int32_t *dst, int32_t *sample,
                    int64_t *hop_offsets);
+/* Order-sensitive 64-bit digest of request req's edge list (src, dst,
+ * sample, position), computed on the device: equal edge lists give equal
+ * digests whichever plan mode or variant produced them. */
+int csaw_plan_digest(const csaw_plan *p, int32_t req, uint64_t *digest);
 /* Average device time per launch of each kernel kind over the runs since
- * the last reset (timing != 0).  names: NUL-separated list; returns count. */
+ * the last reset (timing != 0), and the algorithmic bytes per launch of
+ * that kind (exact per-expansion byte model).  names: NUL-separated list. */
 int csaw_plan_kernel_times(const csaw_plan *p, char *names, size_t names_len, double *avg_ms,
-                           int64_t *launches, int32_t max_kinds, int32_t *n_kinds);
+                           int64_t *launches, double *avg_bytes, int32_t max_kinds, int32_t *n_kinds);
 int csaw_plan_reset_times(csaw_plan *p);
 int csaw_plan_destroy(csaw_plan *p);
 /* Per-chunk cost-model data of the last run (log_chunks != 0): features
