@@ -110,13 +110,26 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--retrain", action="store_true")
-    ap.add_argument("--cpu-sample", type=float, default=0.02)
+    ap.add_argument("--cpu-sample", type=float, default=0.08)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="--only batched: launch eagerly (profilers)")
+    ap.add_argument("--only", choices=["all", "batched", "plain", "copy"], default="all",
+                    help="profiling modes: run one executor only (or a 256 MiB copy for counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported "
                                    "(the path shards by seeds with no exchange; see DESIGN.md)"}))
         sys.exit(2)
+    if a.only == "copy":
+        import torch
+        x = torch.ones(64 << 20, dtype=torch.int32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(a.steps):
+            torch.add(x, 1, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4,
+                          "launches": a.steps}))
+        return
     import __graft_entry__ as ge
     C = ge.load_csaw()
     import cost_model as CM
@@ -129,6 +142,16 @@ def main():
     mlp, meta = get_model(C, CM, g, a.retrain)
     tuning = get_tuning(C, g, specs, mlp, a.retrain)
     log("tuning", {k: getattr(tuning, k) for k, _ in C.Tuning._fields_})
+    if a.only in ("batched", "plain"):
+        if a.only == "plain":
+            p = C.Plan(g, specs, mode=C.MODE_PLAIN)
+        else:
+            p = C.Plan(g, specs, mode=C.MODE_BATCHED, sched=C.SCHED_MLP, mlp=mlp, use_graph=not a.no_graph,
+                       tuning=tuning, cdf_cache=True)
+        t = run_timed(C, p, a.steps, a.warmup)
+        e = sum(p.n_edges(i) for i in range(len(specs)))
+        print(json.dumps({"only": a.only, "ms_per_step": t * 1e3, "edges": e, "value": e / t}))
+        return
 
     plain = C.Plan(g, specs, mode=C.MODE_PLAIN)
     batched = C.Plan(g, specs, mode=C.MODE_BATCHED, sched=C.SCHED_MLP, mlp=mlp, use_graph=True, tuning=tuning,

@@ -259,9 +259,9 @@ __global__ __launch_bounds__(256) void k_expand_plain(GraphDev G, csaw_graph_vie
   const int64_t n = *H.n_ptr;
   const int wid = threadIdx.x / kWave;
   const int64_t nw = (int64_t)gridDim.x * 4;
-  for (int64_t e = blockIdx.x * 4 + wid; e < n; e += nw) {
-    const Entry en = load_entry(G, R, gs2req, H, e);
-    expand_entry_wave<KIND>(G, hv, H, en, KIND == kInt ? cdf + wid * kWaveIntCap : nullptr,
+  for (int64_t e0 = (blockIdx.x * 4 + wid) * (int64_t)kWave; e0 < n; e0 += nw * kWave) {
+    const int cnt = (int)min((int64_t)kWave, n - e0);
+    wave_expand_batch<KIND>(G, hv, R, gs2req, H, nullptr, e0, cnt, KIND == kInt ? cdf + wid * kWaveIntCap : nullptr,
                             bms + wid * (kWaveBitmapCap / 32));
   }
 }
@@ -418,11 +418,13 @@ __global__ __launch_bounds__(256) void k_expand_wave(GraphDev G, csaw_graph_view
     if (!cc.valid) break;
     const uint64_t t0 = wall_clock64();
     const Chunk c = H.chunks[cc.ci];
-    for (int k = wid; k < c.len; k += 4) {
-      const Entry en = load_entry(G, R, gs2req, H, H.order[c.start + k]);
-      expand_entry_wave<KIND>(G, hv, H, en, KIND == kInt ? cdf + wid * kWaveIntCap : nullptr,
+    // each wave takes a contiguous quarter of the chunk, 64 entries per batch
+    const int per = (c.len + 3) / 4;
+    const int k_end = min((int)c.len, (wid + 1) * per);
+    for (int k0 = wid * per; k0 < k_end; k0 += kWave)
+      wave_expand_batch<KIND>(G, hv, R, gs2req, H, H.order, (int64_t)c.start + k0, min(kWave, k_end - k0),
+                              KIND == kInt ? cdf + wid * kWaveIntCap : nullptr,
                               BITMAP ? bms + wid * (kWaveBitmapCap / 32) : nullptr);
-    }
     __syncthreads();
     log_chunk(A, H, cc.ci, var, t0);
   }

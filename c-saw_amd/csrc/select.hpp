@@ -138,14 +138,28 @@ __device__ int32_t wave_select(const Entry& en, const Picker& pk, uint32_t* bm) 
     if (lane < F) res = pk.pick(draw(en, lane, 0));
     return res;
   }
+  int32_t cand = 0;
+  if (lane < F) cand = pk.pick(draw(en, lane, 0));
+  // Fast path: up to the first slot whose attempt-0 draw repeats an earlier
+  // slot's draw, the sequential definition accepts every attempt-0 draw, so
+  // those slots are final; only the rest go through the ordered loop.
+  bool dup = false;
+  for (int i = 0; i + 1 < F; ++i) {
+    const int32_t ci = readlane(cand, i);
+    dup |= (lane > i) & (lane < F) & (cand == ci);
+  }
+  const uint64_t dm = ballot(dup);
+  if (dm == 0ull) return lane < F ? cand : -1;
+  const int j0 = __builtin_ctzll(dm);
+  res = lane < j0 ? cand : -1;
   if (bm) {
     const int words = (d + 31) >> 5;
     for (int i = lane; i < words; i += kWave) bm[i] = 0u;
     wave_fence();
+    if (lane < j0) atomicOr(&bm[cand >> 5], 1u << (cand & 31));
+    wave_fence();
   }
-  int32_t cand = 0;
-  if (lane < F) cand = pk.pick(draw(en, lane, 0));
-  for (int jj = 0; jj < F; ++jj) {
+  for (int jj = j0; jj < F; ++jj) {
     int32_t c = readlane(cand, jj);
     bool taken = bm ? ((bm[c >> 5] >> (c & 31)) & 1u) : (ballot(res == c) != 0ull);
     if (taken) {
@@ -211,24 +225,86 @@ __device__ __forceinline__ void wave_copy_all(const GraphDev& G, const csaw_grap
   for (int i = lane_id(); i < en.d; i += kWave) write_edge(G, hv, H, en, en.pos + i, i);
 }
 
-// Full wave expansion of one entry.  cdf: LDS region of kWaveIntCap u64,
+// Full wave expansion of one entry whose ITS state was prefetched (build:
+// the integer CDF must be built here).  cdf: LDS region of kWaveIntCap u64,
 // bm: LDS bitmap of kWaveBitmapCap bits (nullptr: list taken-set).
 template <int KIND>
 __device__ void expand_entry_wave(const GraphDev& G, const csaw_graph_view& hv, const HopDev& H, const Entry& en,
-                                  uint64_t* cdf, uint32_t* bm) {
+                                  Picker pk, bool build, uint64_t* cdf, uint32_t* bm) {
   if (en.d == 0 || en.F == 0) return;
   if (en.dedup && en.d <= en.F) {
     wave_copy_all(G, hv, H, en);
     return;
   }
-  Picker pk;
-  if (!picker_global(G, en, KIND, kWaveIntCap, pk)) {
+  if (build) {
     pk.C = cdf;
     pk.T = wave_build_int_cdf(G, hv, en, cdf);
     pk.mode = pk.T > 0 ? 1 : 0;
   }
   const int32_t res = wave_select(en, pk, (bm && en.d <= kWaveBitmapCap) ? bm : nullptr);
   if (lane_id() < en.F) write_edge(G, hv, H, en, en.pos + lane_id(), res);
+}
+
+__device__ __forceinline__ int64_t readlane_i64(int64_t v, int l) { return (int64_t)readlane_u64((uint64_t)v, l); }
+
+__device__ __forceinline__ Entry bcast_entry(const Entry& m, int j) {
+  Entry e;
+  e.v = readlane(m.v, j);
+  e.d = readlane(m.d, j);
+  e.o = readlane_i64(m.o, j);
+  e.F = readlane(m.F, j);
+  e.dedup = readlane(m.dedup, j);
+  e.bias = readlane(m.bias, j);
+  e.g = readlane(m.g, j);
+  e.pos = readlane_i64(m.pos, j);
+  e.c1 = readlane_u(m.c1, j);
+  e.c2 = readlane_u(m.c2, j);
+  e.c3 = readlane_u(m.c3, j);
+  e.k0 = readlane_u(m.k0, j);
+  e.k1 = readlane_u(m.k1, j);
+  return e;
+}
+
+__device__ __forceinline__ Picker bcast_picker(const Picker& m, int j) {
+  Picker p;
+  p.mode = readlane(m.mode, j);
+  p.d = readlane(m.d, j);
+  p.T = readlane_u64(m.T, j);
+  p.Tf = __int_as_float(readlane(__float_as_int(m.Tf), j));
+  p.C = (const uint64_t*)readlane_u64((uint64_t)m.C, j);
+  p.Cf = (const float*)readlane_u64((uint64_t)m.Cf, j);
+  return p;
+}
+
+// Expands `count` (<= 64) entries with one wave: lane i loads entry i's
+// metadata and ITS state (one round of dependent loads for all of them),
+// then the wave expands the entries in order from broadcast values.
+template <int KIND>
+__device__ void wave_expand_batch(const GraphDev& G, const csaw_graph_view& hv, const ReqDev* __restrict__ R,
+                                  const int32_t* __restrict__ gs2req, const HopDev& H,
+                                  const int32_t* __restrict__ order, int64_t first, int count, uint64_t* cdf,
+                                  uint32_t* bm) {
+  const int lane = lane_id();
+  Entry mine;
+  Picker pm;
+  pm.mode = 0;
+  pm.d = 0;
+  pm.T = 0;
+  pm.Tf = 0.f;
+  pm.C = nullptr;
+  pm.Cf = nullptr;
+  int build = 0;
+  mine.d = 0;
+  if (lane < count) {
+    mine = load_entry(G, R, gs2req, H, order ? (int64_t)order[first + lane] : first + lane);
+    if (mine.d > 0 && mine.F > 0 && !(mine.dedup && mine.d <= mine.F))
+      build = picker_global(G, mine, KIND, kWaveIntCap, pm) ? 0 : 1;
+  }
+  for (int j = 0; j < count; ++j) {
+    const Entry en = bcast_entry(mine, j);
+    const Picker pk = bcast_picker(pm, j);
+    expand_entry_wave<KIND>(G, hv, H, en, pk, readlane(build, j) != 0, cdf, bm);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -281,8 +357,76 @@ __device__ void expand_entry_thread(const GraphDev& G, const csaw_graph_view& hv
     }
     return i;
   };
+  // Four slots at a time: their attempt-0 searches advance in lockstep, so
+  // each lane keeps up to four independent loads in flight.
+  constexpr int G4 = 4;
+  auto pick4 = [&](int j0, int nq, int32_t* out) {
+    if (mode == 2 || mode == 3) {
+      int32_t lo[G4], hi[G4];
+      uint64_t xi[G4];
+      float xf[G4];
+#pragma unroll
+      for (int q = 0; q < G4; ++q) {
+        lo[q] = 0;
+        hi[q] = q < nq ? d : 0;
+        xi[q] = 0;
+        xf[q] = 0.f;
+        if (q < nq) {
+          const U4 o = draw(en, j0 + q, 0);
+          if (mode == 2) xi[q] = mulhi64(philox_t64(o), T);
+          else xf[q] = __fmul_rn(philox_u24(o), Tf);
+        }
+      }
+      for (;;) {
+        bool any = false;
+        uint64_t vi[G4];
+        float vf[G4];
+        int32_t mid[G4];
+#pragma unroll
+        for (int q = 0; q < G4; ++q) {
+          mid[q] = (lo[q] + hi[q]) >> 1;
+          if (lo[q] < hi[q]) {
+            any = true;
+            if (mode == 2) vi[q] = Cg[mid[q]];
+            else vf[q] = Cf[mid[q]];
+          }
+        }
+        if (!any) break;
+#pragma unroll
+        for (int q = 0; q < G4; ++q) {
+          if (lo[q] < hi[q]) {
+            const bool gt = mode == 2 ? vi[q] > xi[q] : vf[q] > xf[q];
+            if (gt) hi[q] = mid[q]; else lo[q] = mid[q] + 1;
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < G4; ++q) out[q] = lo[q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < G4; ++q) out[q] = q < nq ? pick(draw(en, j0 + q, 0)) : 0;
+    }
+  };
+  auto emit4 = [&](int j0, int nq, const int32_t* idx) {
+    int32_t nb[G4];
+#pragma unroll
+    for (int q = 0; q < G4; ++q) nb[q] = q < nq ? G.adj[en.o + idx[q]] : 0;
+#pragma unroll
+    for (int q = 0; q < G4; ++q)
+      if (q < nq) {
+        const int64_t p = en.pos + j0 + q;
+        H.src[p] = en.v;
+        H.dst[p] = Update(&hv, nb[q], en.v);
+        H.smp[p] = en.g;
+      }
+  };
   if (!en.dedup) {
-    for (int j = 0; j < en.F; ++j) write_edge(G, hv, H, en, en.pos + j, pick(draw(en, j, 0)));
+    for (int j0 = 0; j0 < en.F; j0 += G4) {
+      const int nq = min(G4, en.F - j0);
+      int32_t idx[G4];
+      pick4(j0, nq, idx);
+      emit4(j0, nq, idx);
+    }
     return;
   }
   const bool use_mask = d <= 64;
@@ -294,24 +438,34 @@ __device__ void expand_entry_thread(const GraphDev& G, const csaw_graph_view& hv
       if ((int32_t)lds[m * stride] == i) return true;
     return false;
   };
-  for (int j = 0; j < en.F; ++j) {
-    int32_t i = 0;
-    bool found = false;
-    for (int a = 0; a < kReselect; ++a) {
-      i = pick(draw(en, j, a));
-      if (!taken(i)) {
-        found = true;
-        break;
+  for (int j0 = 0; j0 < en.F; j0 += G4) {
+    const int nq = min(G4, en.F - j0);
+    int32_t idx[G4];
+    pick4(j0, nq, idx);
+#pragma unroll
+    for (int q = 0; q < G4; ++q) {
+      if (q >= nq) break;
+      int32_t i = idx[q];
+      if (taken(i)) {
+        bool found = false;
+        for (int a = 1; a < kReselect; ++a) {
+          i = pick(draw(en, j0 + q, a));
+          if (!taken(i)) {
+            found = true;
+            break;
+          }
+        }
+        if (!found) {
+          do {
+            i = (i + 1 == d) ? 0 : i + 1;
+          } while (taken(i));
+        }
       }
+      if (use_mask) mask |= 1ull << i;
+      else lds[(n_taken++) * stride] = (uint32_t)i;
+      idx[q] = i;
     }
-    if (!found) {
-      do {
-        i = (i + 1 == d) ? 0 : i + 1;
-      } while (taken(i));
-    }
-    if (use_mask) mask |= 1ull << i;
-    else lds[(n_taken++) * stride] = (uint32_t)i;
-    write_edge(G, hv, H, en, en.pos + j, i);
+    emit4(j0, nq, idx);
   }
 }
 
