@@ -104,14 +104,151 @@ def cpu_baseline(C, O, g, specs, gpu_plan, frac, nthreads):
     return edges, secs, mism, checked
 
 
+def roofline_of(plan, runs=2):
+    """Dominant kernel kind of an event-timed plan: algorithmic bytes per launch / avg launch time."""
+    plan.run()
+    plan.reset_times()
+    for _ in range(runs):
+        plan.run()
+    kt = plan.kernel_times(with_bytes=True)
+    dom = max(kt, key=lambda k: kt[k][0] * kt[k][1])
+    avg_ms, nl, avg_b = kt[dom]
+    ach = avg_b / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": None, "kernel": dom, "algo_bytes_per_launch": avg_b, "avg_launch_ms": avg_ms}, kt
+
+
+def line(metric, value, unit, steps, warmup, ms, config, dtype, roof, cpu, **extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+           "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+           "data": "synthetic", "config": config, "roofline": roof, "cpu_baseline": cpu}
+    out.update(extra)
+    print(json.dumps(out))
+
+
+def oracle_check_sample(O, og, specs, gpu_results, frac, nthreads):
+    """Oracle on the first ceil(frac*S) samples of each request; returns
+    (edges, seconds, request-hops equal, request-hops checked)."""
+    edges, secs, ok, checked = 0, 0.0, 0, 0
+    for i, s in enumerate(specs):
+        m = max(1, int(np.ceil(frac * len(s["seeds"]))))
+        t0 = time.perf_counter()
+        r = O.sample(og, s["seeds"][:m], s["fanouts"], bias=s["bias"], request=s.get("request", i),
+                     nthreads=nthreads)
+        secs += time.perf_counter() - t0
+        edges += len(r["src"])
+        full = gpu_results(i)
+        for h in range(len(s["fanouts"])):
+            a, b = full["hop_off"][h, 0], full["hop_off"][h, m]
+            ra, rb = r["hop_off"][h, 0], r["hop_off"][h, m]
+            checked += 1
+            ok += int(np.array_equal(full["src"][a:b], r["src"][ra:rb])
+                      and np.array_equal(full["dst"][a:b], r["dst"][ra:rb]))
+    return edges, secs, ok, checked
+
+
+def bench_single(a, C, O, name, scale, ef, weights, seeds_n, fanouts, bias, variant=None):
+    """C1 / C2 / plain-uniform instance: one request, plain and batched executors."""
+    g = C.Graph.rmat(scale, ef, weights=weights)
+    specs = [{"seeds": g.seeds(seeds_n), "fanouts": fanouts, "bias": bias, "request": 0}]
+    plain = C.Plan(g, specs, mode=C.MODE_PLAIN)
+    sched = C.SCHED_FORCED if variant is not None else C.SCHED_HEURISTIC
+    bat = C.Plan(g, specs, mode=C.MODE_BATCHED, sched=sched, variant=variant or 0, use_graph=True, cdf_cache=True)
+    tp = run_timed(C, plain, a.steps, a.warmup)
+    tb = run_timed(C, bat, a.steps, a.warmup)
+    e = plain.n_edges(0)
+    assert plain.digest(0) == bat.digest(0)
+    roof, kt = roofline_of(C.Plan(g, specs, mode=C.MODE_PLAIN, timing=True))
+    cpu = None
+    if not a.no_cpu:
+        off, adj, *w = g.csr(weights=weights)
+        og = O.OracleGraph(off, adj, weights=w[0] if w else None)
+        ce, cs, ok, chk = oracle_check_sample(O, og, specs, plain.result, a.cpu_sample if seeds_n > 10000 else 1.0,
+                                              min(16, os.cpu_count() or 1))
+        cpu = {"value": ce / cs, "unit": "edges/s", "cores": min(16, os.cpu_count() or 1), "kind": "port",
+               "sample": f"{ce} edges ({cs:.1f} s); oracle == GPU in {ok}/{chk} hops"}
+    line(f"SEPS ({name})", e / tp, "edges/s", a.steps, a.warmup, tp * 1e3,
+         {"workload": name, "seeds": seeds_n, "fanouts": fanouts, "bias": bias, "edges_per_step": e},
+         "u64", roof, cpu, batched_value=e / tb, kernel_times={k: v[0] for k, v in kt.items()})
+
+
+def bench_walk(a, C, O):
+    """C3: RMAT-22 ef16 FP32 weights, 1,048,576 walkers x 80 steps."""
+    g = C.Graph.rmat(22, 16, weights=True)
+    seeds = g.seeds(1 << 20)
+    wp = C.WalkPlan(g, seeds, 80, bias=C.BIAS_WEIGHT)
+    for _ in range(a.warmup):
+        wp.run()
+    ms = []
+    for _ in range(a.steps):
+        wp.run()
+        W, steps, kms = wp.result(walks=False)
+        ms.append(kms)
+    kms = float(np.median(ms))
+    ach = wp.algo_bytes / (kms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": None, "kernel": "walk", "algo_bytes_per_launch": wp.algo_bytes, "avg_launch_ms": kms}
+    cpu = None
+    if not a.no_cpu:
+        W, _, _ = wp.result()
+        off, adj, w = g.csr(weights=True)
+        og = O.OracleGraph(off, adj, weights=w)
+        m = int(len(seeds) * a.cpu_sample)
+        t0 = time.perf_counter()
+        Wo, so = O.walk(og, seeds[:m], 80, bias=O.BIAS_WEIGHT, nthreads=min(16, os.cpu_count() or 1))
+        cs = time.perf_counter() - t0
+        cpu = {"value": so / cs, "unit": "steps/s", "cores": min(16, os.cpu_count() or 1), "kind": "port",
+               "sample": f"first {m} walkers ({so} steps, {cs:.1f} s); equal to the GPU rows: "
+                         f"{bool(np.array_equal(Wo, W[:m]))}"}
+    line("walk steps/s (C3)", steps / (kms * 1e-3), "steps/s", a.steps, a.warmup, kms,
+         {"workload": "RMAT-22 ef16, FP32 weights, 1,048,576 walkers x 80", "steps": int(steps)}, "f32", roof, cpu)
+
+
+def bench_stream(a, C, O):
+    """C5: RMAT-26 ef16 in pinned host memory, 4 GB device budget, 16 partitions,
+    65,536 seeds, fanouts {15,10,5}, degree bias."""
+    t0 = time.time()
+    hg = C.HostGraph.rmat(26, 16)
+    log(f"host graph V={hg.V} E={hg.E} maxdeg={hg.max_degree} ({time.time() - t0:.1f}s)")
+    seeds = C.seeds_v(hg.V, 65536)
+    res = None
+    ms = []
+    for i in range(a.warmup + a.steps):
+        res = C.stream_sample(hg, seeds, [15, 10, 5], bias=C.BIAS_DEGREE, budget_bytes=4 << 30, partitions=16)
+        if i >= a.warmup:
+            ms.append(res["stats"].device_ms)
+    st = res["stats"]
+    dms = float(np.median(ms))
+    cpu = None
+    if not a.no_cpu:
+        off, adj = hg.csr_views()
+        og = O.OracleGraph(off, adj)
+        m = 256
+        t1 = time.perf_counter()
+        r = O.sample(og, seeds[:m], [15, 10, 5], bias=O.BIAS_DEGREE, nthreads=min(16, os.cpu_count() or 1))
+        cs = time.perf_counter() - t1
+        okh = all(np.array_equal(res["dst"][res["hop_off"][h, 0]:res["hop_off"][h, m]],
+                                 r["dst"][r["hop_off"][h, 0]:r["hop_off"][h, m]]) for h in range(3))
+        cpu = {"value": len(r["src"]) / cs, "unit": "edges/s", "cores": min(16, os.cpu_count() or 1),
+               "kind": "port", "sample": f"first {m} seeds ({len(r['src'])} edges, {cs:.1f} s); equal to the "
+                                         f"streamed GPU edges: {okh}"}
+    line("SEPS streaming (C5)", st.edges / (dms * 1e-3), "edges/s", a.steps, a.warmup, dms,
+         {"workload": "RMAT-26 ef16 host-resident, 4 GB budget, 16 partitions, 65,536 seeds, {15,10,5}, degree",
+          "edges": int(st.edges), "h2d_bytes": st.h2d_bytes, "peak_device_bytes": int(st.peak_device_bytes),
+          "slice_bytes": int(st.slice_bytes)}, "u64", None, cpu,
+         h2d_gbs=st.h2d_bytes / (dms * 1e-3) / 1e9)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--retrain", action="store_true")
-    ap.add_argument("--cpu-sample", type=float, default=0.08)
+    ap.add_argument("--cpu-sample", type=float, default=0.25)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["c4", "c1", "c2", "c3", "uniform", "c5"], default="c4",
+                    help="c4 (default) is the headline; the others measure the remaining configs")
     ap.add_argument("--no-graph", action="store_true", help="--only batched: launch eagerly (profilers)")
     ap.add_argument("--only", choices=["all", "batched", "plain", "copy"], default="all",
                     help="profiling modes: run one executor only (or a 256 MiB copy for counter calibration)")
@@ -133,6 +270,23 @@ def main():
     import __graft_entry__ as ge
     C = ge.load_csaw()
     import cost_model as CM
+
+    if a.workload != "c4":
+        O = ge.load_oracle() if not a.no_cpu else None
+        if a.workload == "c1":
+            bench_single(a, C, O, "C1: RMAT-16 ef8, 1000 samples, FS 1, NS 2, depth 2, uniform, bitmap",
+                         16, 8, False, 1000, [2, 2], C.BIAS_UNIFORM, variant=C.VAR_WAVE_BITMAP)
+        elif a.workload == "c2":
+            bench_single(a, C, O, "C2: RMAT-20 ef5, 8192 seeds, {25,10}, degree bias", 20, 5, False, 8192, [25, 10],
+                         C.BIAS_DEGREE)
+        elif a.workload == "uniform":
+            bench_single(a, C, O, "plain uniform: RMAT-22 ef16, 1,048,576 seeds, {25,10}, uniform", 22, 16, False,
+                         1 << 20, [25, 10], C.BIAS_UNIFORM)
+        elif a.workload == "c3":
+            bench_walk(a, C, O)
+        else:
+            bench_stream(a, C, O)
+        return
 
     t0 = time.time()
     g = C.Graph.rmat(22, 16, weights=True)

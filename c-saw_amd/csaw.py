@@ -427,3 +427,109 @@ class Mlp:
             self.close()
         except Exception:
             pass
+
+
+# --------------------------------------------------------------- streaming
+class StreamStats(ctypes.Structure):
+    _fields_ = [("edges", i64), ("device_ms", ctypes.c_double), ("h2d_bytes", ctypes.c_double),
+                ("partitions", i32), ("peak_device_bytes", i64), ("slice_bytes", i64)]
+
+
+_sig("csaw_host_graph_create_rmat", [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                     u64, P])
+_sig("csaw_host_graph_from_csr", [P, P, i64, i64, P])
+_sig("csaw_host_graph_info", [P, P, P, P, P, P])
+_sig("csaw_host_graph_destroy", [P])
+_sig("csaw_stream_sample", [P, P, i64, i32, P, P])
+_sig("csaw_edges_from_host", [i32, i32, i64, P, P, P, P, P])
+_sig("csaw_mdrw", [P, u64, u32, i32, i32, i32, P, P, P, P, P])
+
+
+class HostGraph:
+    """CSR in pinned host memory for the streaming sampler."""
+
+    def __init__(self, handle):
+        self.h = handle
+        V, E, md = i64(), i64(), i64()
+        po, pa = P(), P()
+        _check(_L.csaw_host_graph_info(self.h, ctypes.byref(V), ctypes.byref(E), ctypes.byref(md), ctypes.byref(po),
+                                       ctypes.byref(pa)), "host_graph_info")
+        self.V, self.E, self.max_degree = V.value, E.value, md.value
+        self._po, self._pa = po.value, pa.value
+
+    @classmethod
+    def rmat(cls, scale, edge_factor, a=0.57, b=0.19, c=0.19, key=42):
+        h = P()
+        _check(_L.csaw_host_graph_create_rmat(scale, edge_factor, a, b, c, key, ctypes.byref(h)), "host_graph_rmat")
+        return cls(h)
+
+    @classmethod
+    def from_csr(cls, off, adj):
+        off = np.ascontiguousarray(off, dtype=np.int64)
+        adj = np.ascontiguousarray(adj, dtype=np.int32)
+        h = P()
+        _check(_L.csaw_host_graph_from_csr(_p(off), _p(adj), len(off) - 1, len(adj), ctypes.byref(h)),
+               "host_graph_from_csr")
+        return cls(h)
+
+    def csr_views(self):
+        """Zero-copy numpy views of the pinned CSR (valid while the graph lives)."""
+        off = np.ctypeslib.as_array(ctypes.cast(self._po, ctypes.POINTER(ctypes.c_int64)), (self.V + 1,))
+        adj = np.ctypeslib.as_array(ctypes.cast(self._pa, ctypes.POINTER(ctypes.c_int32)), (max(self.E, 1),))
+        return off, adj[:self.E]
+
+    def close(self):
+        if self.h:
+            _L.csaw_host_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def stream_sample(hg, seeds, fanouts, bias=BIAS_DEGREE, dedup=1, key=42, request=0, budget_bytes=4 << 30,
+                  partitions=16):
+    reqs, keep = make_requests([{"seeds": seeds, "fanouts": fanouts, "bias": bias, "dedup": dedup, "key": key,
+                                 "request": request}])
+    e = P()
+    st = StreamStats()
+    _check(_L.csaw_stream_sample(hg.h, reqs, budget_bytes, partitions, ctypes.byref(e), ctypes.byref(st)),
+           "stream_sample")
+    try:
+        n, d, s = i64(), i32(), i32()
+        _check(_L.csaw_edges_info(e, ctypes.byref(n), ctypes.byref(d), ctypes.byref(s)), "edges_info")
+        src = np.zeros(max(n.value, 1), np.int32)
+        dst = np.zeros(max(n.value, 1), np.int32)
+        smp = np.zeros(max(n.value, 1), np.int32)
+        ho = np.zeros(max(d.value * (s.value + 1), 1), np.int64)
+        _check(_L.csaw_edges_copy(e, _p(src), _p(dst), _p(smp), _p(ho)), "edges_copy")
+    finally:
+        _L.csaw_edges_free(e)
+    k = n.value
+    return {"src": src[:k], "dst": dst[:k], "smp": smp[:k],
+            "hop_off": ho[:d.value * (s.value + 1)].reshape(d.value, s.value + 1), "stats": st}
+
+
+def mdrw(g, seeds, samples, frontier_size, depth, key=42, request=0):
+    seeds = np.ascontiguousarray(seeds, dtype=np.int32)
+    src = np.zeros(max(samples * depth, 1), np.int32)
+    dst = np.zeros(max(samples * depth, 1), np.int32)
+    off = np.zeros(samples + 1, np.int64)
+    st = Stats()
+    _check(_L.csaw_mdrw(g.h, key, request, samples, frontier_size, depth, _p(seeds), _p(src), _p(dst), _p(off),
+                        ctypes.byref(st)), "mdrw")
+    n = int(off[-1])
+    return src[:n], dst[:n], off
+
+
+_sig("csaw_seeds_v", [i64, u64, u32, i64, P])
+
+
+def seeds_v(V, n, key=42, request=0):
+    """Seeds for a vertex count V, computed on the host (same values as Graph.seeds)."""
+    out = np.zeros(max(n, 1), np.int32)
+    _check(_L.csaw_seeds_v(V, key, request, n, _p(out)), "seeds_v")
+    return out[:n]

@@ -283,12 +283,23 @@ int graph_prepare_bias(csaw_graph* g, int32_t bias) {
     CSAW_HIP(hipMemcpy(g->hub_pool, pool.data(), sizeof(int64_t) * pool.size(), hipMemcpyHostToDevice));
   }
   if (g->hub_cdf[kind] || g->n_hubs == 0) return CSAW_OK;
-  // segment work list
   std::vector<int64_t> pool(g->n_hubs + 1);
   CSAW_HIP(hipMemcpy(pool.data(), g->hub_pool, sizeof(int64_t) * pool.size(), hipMemcpyDeviceToHost));
+  CSAW_HIP(hipMalloc(&g->hub_cdf[kind], sizeof(uint64_t) * g->hub_entries));
+  CSAW_TRY(hub_cdf_build(g->view(), g->hook_view(), kind, pool, g->hub_cdf[kind], g->stream, g->num_cu));
+  CSAW_HIP(hipStreamSynchronize(g->stream));
+  return CSAW_OK;
+}
+
+// Builds the inclusive integer CDFs of the hubs G.hub_vid[0..n) into out
+// (pool offsets host_pool, n+1 entries), each hub split across workgroups.
+int hub_cdf_build(const GraphDev& G, const csaw_graph_view& hv, int kind, const std::vector<int64_t>& pool,
+                  uint64_t* out, hipStream_t stream, int num_cu) {
+  const int32_t n_hubs = (int32_t)pool.size() - 1;
+  if (n_hubs <= 0) return CSAW_OK;
   std::vector<int32_t> seg_hub, seg_idx;
   std::vector<int64_t> seg_first{0};
-  for (int32_t h = 0; h < g->n_hubs; ++h) {
+  for (int32_t h = 0; h < n_hubs; ++h) {
     int64_t d = pool[h + 1] - pool[h];
     int64_t ns = (d + kHubSeg - 1) / kHubSeg;
     for (int64_t s = 0; s < ns; ++s) {
@@ -301,27 +312,25 @@ int graph_prepare_bias(csaw_graph* g, int32_t bias) {
   int32_t *d_seg_hub, *d_seg_idx;
   int64_t* d_seg_first;
   uint64_t* d_seg_tot;
-  CSAW_HIP(hipMalloc(&g->hub_cdf[kind], sizeof(uint64_t) * g->hub_entries));
   CSAW_HIP(hipMalloc(&d_seg_hub, sizeof(int32_t) * n_seg));
   CSAW_HIP(hipMalloc(&d_seg_idx, sizeof(int32_t) * n_seg));
   CSAW_HIP(hipMalloc(&d_seg_first, sizeof(int64_t) * seg_first.size()));
   CSAW_HIP(hipMalloc(&d_seg_tot, sizeof(uint64_t) * n_seg));
-  CSAW_HIP(hipMemcpy(d_seg_hub, seg_hub.data(), sizeof(int32_t) * n_seg, hipMemcpyHostToDevice));
-  CSAW_HIP(hipMemcpy(d_seg_idx, seg_idx.data(), sizeof(int32_t) * n_seg, hipMemcpyHostToDevice));
-  CSAW_HIP(hipMemcpy(d_seg_first, seg_first.data(), sizeof(int64_t) * seg_first.size(), hipMemcpyHostToDevice));
-  GraphDev G = g->view();
-  csaw_graph_view hv = g->hook_view();
-  const int grid = (int)std::min<int64_t>(n_seg, (int64_t)g->num_cu * 8);
+  CSAW_HIP(hipMemcpyAsync(d_seg_hub, seg_hub.data(), sizeof(int32_t) * n_seg, hipMemcpyHostToDevice, stream));
+  CSAW_HIP(hipMemcpyAsync(d_seg_idx, seg_idx.data(), sizeof(int32_t) * n_seg, hipMemcpyHostToDevice, stream));
+  CSAW_HIP(hipMemcpyAsync(d_seg_first, seg_first.data(), sizeof(int64_t) * seg_first.size(), hipMemcpyHostToDevice,
+                          stream));
+  const int grid = (int)std::min<int64_t>(n_seg, (int64_t)num_cu * 8);
   if (kind == 0)
-    k_hub_segments<0><<<grid, kHubThreads, 0, g->stream>>>(G, hv, d_seg_hub, d_seg_idx, n_seg, g->hub_cdf[kind], d_seg_tot);
+    k_hub_segments<0><<<grid, kHubThreads, 0, stream>>>(G, hv, d_seg_hub, d_seg_idx, n_seg, out, d_seg_tot);
   else
-    k_hub_segments<1><<<grid, kHubThreads, 0, g->stream>>>(G, hv, d_seg_hub, d_seg_idx, n_seg, g->hub_cdf[kind], d_seg_tot);
+    k_hub_segments<1><<<grid, kHubThreads, 0, stream>>>(G, hv, d_seg_hub, d_seg_idx, n_seg, out, d_seg_tot);
   CSAW_LAUNCH_CHECK();
-  k_hub_seg_offsets<<<grid_for(g->n_hubs, 64, g->num_cu), 64, 0, g->stream>>>(d_seg_first, g->n_hubs, d_seg_tot);
+  k_hub_seg_offsets<<<grid_for(n_hubs, 64, num_cu), 64, 0, stream>>>(d_seg_first, n_hubs, d_seg_tot);
   CSAW_LAUNCH_CHECK();
-  k_hub_add<<<grid, kHubThreads, 0, g->stream>>>(G, d_seg_hub, d_seg_idx, n_seg, d_seg_tot, g->hub_cdf[kind]);
+  k_hub_add<<<grid, kHubThreads, 0, stream>>>(G, d_seg_hub, d_seg_idx, n_seg, d_seg_tot, out);
   CSAW_LAUNCH_CHECK();
-  CSAW_HIP(hipStreamSynchronize(g->stream));
+  CSAW_HIP(hipStreamSynchronize(stream));
   CSAW_HIP(hipFree(d_seg_hub));
   CSAW_HIP(hipFree(d_seg_idx));
   CSAW_HIP(hipFree(d_seg_first));
@@ -496,6 +505,59 @@ static uint32_t thr32(double x) {
   return (uint32_t)(x * 4294967296.0);
 }
 
+namespace csaw {
+// GPU RMAT: Philox pairs, both directions, radix sort, unique, CSR.
+int rmat_generate_device(int scale, int edge_factor, double a, double b, double c, uint64_t key, hipStream_t stream,
+                         int num_cu, int64_t* V_out, int64_t* E_out, int64_t** d_off, int32_t** d_adj) {
+  const int64_t V = (int64_t)1 << scale;
+  const int64_t M = (int64_t)edge_factor << scale;
+  const size_t n = (size_t)(2 * M);
+  uint64_t *k0, *k1;
+  CSAW_HIP(hipMalloc(&k0, sizeof(uint64_t) * n));
+  CSAW_HIP(hipMalloc(&k1, sizeof(uint64_t) * n));
+  k_rmat_pairs<<<grid_for(M, 256, num_cu), 256, 0, stream>>>(scale, M, thr32(a), thr32(a + b), thr32(a + b + c),
+                                                             (uint32_t)key, (uint32_t)(key >> 32), k0);
+  CSAW_LAUNCH_CHECK();
+  rocprim::double_buffer<uint64_t> db(k0, k1);
+  size_t tmp_bytes = 0;
+  CSAW_HIP(rocprim::radix_sort_keys(nullptr, tmp_bytes, db, n, 0, 2 * scale, stream));
+  void* tmp;
+  CSAW_HIP(hipMalloc(&tmp, tmp_bytes));
+  CSAW_HIP(rocprim::radix_sort_keys(tmp, tmp_bytes, db, n, 0, 2 * scale, stream));
+  CSAW_HIP(hipStreamSynchronize(stream));
+  CSAW_HIP(hipFree(tmp));
+  uint64_t* sorted = db.current();
+  uint64_t* uniq = db.alternate();
+  uint32_t* d_cnt;
+  CSAW_HIP(hipMalloc(&d_cnt, sizeof(uint32_t)));
+  tmp_bytes = 0;
+  CSAW_HIP(rocprim::unique(nullptr, tmp_bytes, sorted, uniq, d_cnt, n, rocprim::equal_to<uint64_t>(), stream));
+  CSAW_HIP(hipMalloc(&tmp, tmp_bytes));
+  CSAW_HIP(rocprim::unique(tmp, tmp_bytes, sorted, uniq, d_cnt, n, rocprim::equal_to<uint64_t>(), stream));
+  uint32_t cnt = 0;
+  CSAW_HIP(hipMemcpyAsync(&cnt, d_cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+  CSAW_HIP(hipStreamSynchronize(stream));
+  CSAW_HIP(hipFree(tmp));
+  CSAW_HIP(hipFree(d_cnt));
+  int64_t E = cnt;
+  if (E > 0) {
+    uint64_t last = 0;
+    CSAW_HIP(hipMemcpy(&last, uniq + E - 1, sizeof(uint64_t), hipMemcpyDeviceToHost));
+    if (last == ((1ull << (2 * scale)) - 1)) --E;  // self-loop sentinel
+  }
+  CSAW_HIP(hipMalloc(d_off, sizeof(int64_t) * (V + 1)));
+  CSAW_HIP(hipMalloc(d_adj, sizeof(int32_t) * (E > 0 ? E : 1)));
+  k_keys_to_csr<<<grid_for(E + 1, 256, num_cu), 256, 0, stream>>>(uniq, E, scale, V, *d_adj, *d_off);
+  CSAW_LAUNCH_CHECK();
+  CSAW_HIP(hipStreamSynchronize(stream));
+  CSAW_HIP(hipFree(k0));
+  CSAW_HIP(hipFree(k1));
+  *V_out = V;
+  *E_out = E;
+  return CSAW_OK;
+}
+}  // namespace csaw
+
 extern "C" int csaw_graph_create_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t key,
                                       int with_weights, csaw_graph** out) {
   if (!out || scale < 1 || scale > 30 || edge_factor < 1 || a < 0 || b < 0 || c < 0 || a + b + c > 1.0) {
@@ -507,50 +569,9 @@ extern "C" int csaw_graph_create_rmat(int scale, int edge_factor, double a, doub
   int st = graph_alloc_common(g);
   if (st != CSAW_OK) { graph_free(g); return st; }
   auto body = [&]() -> int {
-    const int64_t V = (int64_t)1 << scale;
-    const int64_t M = (int64_t)edge_factor << scale;
-    const size_t n = (size_t)(2 * M);
-    uint64_t *k0, *k1;
-    CSAW_HIP(hipMalloc(&k0, sizeof(uint64_t) * n));
-    CSAW_HIP(hipMalloc(&k1, sizeof(uint64_t) * n));
-    k_rmat_pairs<<<grid_for(M, 256, g->num_cu), 256, 0, g->stream>>>(scale, M, thr32(a), thr32(a + b), thr32(a + b + c),
-                                                                     (uint32_t)key, (uint32_t)(key >> 32), k0);
-    CSAW_LAUNCH_CHECK();
-    rocprim::double_buffer<uint64_t> db(k0, k1);
-    size_t tmp_bytes = 0;
-    CSAW_HIP(rocprim::radix_sort_keys(nullptr, tmp_bytes, db, n, 0, 2 * scale, g->stream));
-    void* tmp;
-    CSAW_HIP(hipMalloc(&tmp, tmp_bytes));
-    CSAW_HIP(rocprim::radix_sort_keys(tmp, tmp_bytes, db, n, 0, 2 * scale, g->stream));
-    CSAW_HIP(hipFree(tmp));
-    uint64_t* sorted = db.current();
-    uint64_t* uniq = db.alternate();
-    uint32_t* d_cnt;
-    CSAW_HIP(hipMalloc(&d_cnt, sizeof(uint32_t)));
-    tmp_bytes = 0;
-    CSAW_HIP(rocprim::unique(nullptr, tmp_bytes, sorted, uniq, d_cnt, n, rocprim::equal_to<uint64_t>(), g->stream));
-    CSAW_HIP(hipMalloc(&tmp, tmp_bytes));
-    CSAW_HIP(rocprim::unique(tmp, tmp_bytes, sorted, uniq, d_cnt, n, rocprim::equal_to<uint64_t>(), g->stream));
-    uint32_t cnt = 0;
-    CSAW_HIP(hipMemcpyAsync(&cnt, d_cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
-    CSAW_HIP(hipStreamSynchronize(g->stream));
-    CSAW_HIP(hipFree(tmp));
-    CSAW_HIP(hipFree(d_cnt));
-    int64_t E = cnt;
-    if (E > 0) {
-      uint64_t last = 0;
-      CSAW_HIP(hipMemcpy(&last, uniq + E - 1, sizeof(uint64_t), hipMemcpyDeviceToHost));
-      if (last == ((1ull << (2 * scale)) - 1)) --E;  // self-loop sentinel
-    }
-    g->V = V;
-    g->E = E;
-    CSAW_HIP(hipMalloc(&g->off, sizeof(int64_t) * (V + 1)));
-    CSAW_HIP(hipMalloc(&g->adj, sizeof(int32_t) * (E > 0 ? E : 1)));
-    k_keys_to_csr<<<grid_for(E + 1, 256, g->num_cu), 256, 0, g->stream>>>(uniq, E, scale, V, g->adj, g->off);
-    CSAW_LAUNCH_CHECK();
-    CSAW_HIP(hipStreamSynchronize(g->stream));
-    CSAW_HIP(hipFree(k0));
-    CSAW_HIP(hipFree(k1));
+    CSAW_TRY(rmat_generate_device(scale, edge_factor, a, b, c, key, g->stream, g->num_cu, &g->V, &g->E, &g->off,
+                                  &g->adj));
+    const int64_t E = g->E;
     if (with_weights) {
       CSAW_HIP(hipMalloc(&g->w, sizeof(float) * (E > 0 ? E : 1)));
       if (E > 0) {
@@ -659,3 +680,14 @@ extern "C" int csaw_seeds(const csaw_graph* g, uint64_t key, uint32_t request, i
 }
 
 extern "C" const char* csaw_last_error(void) { return csaw::last_error(); }
+
+// Host-side seeds for graphs that only live in host memory.
+extern "C" int csaw_seeds_v(int64_t V, uint64_t key, uint32_t request, int64_t n, int32_t* out) {
+  if (V < 1 || n < 0 || (n > 0 && !out)) return CSAW_ERR_INVALID_ARG;
+  for (int64_t i = 0; i < n; ++i) {
+    const U4 o = philox4x32_10((uint32_t)i, (uint32_t)(i >> 32), request, kDomSeeds << 28, (uint32_t)key,
+                               (uint32_t)(key >> 32));
+    out[i] = (int32_t)mulhi64(philox_t64(o), (uint64_t)V);
+  }
+  return CSAW_OK;
+}

@@ -32,6 +32,10 @@ namespace csaw {
 // integer hub cache (CSAW_BIAS_DEGREE / CSAW_BIAS_USER).
 int graph_prepare_bias(csaw_graph* g, int32_t bias);
 int graph_prepare_int_cache(csaw_graph* g, int32_t bias);
+int hub_cdf_build(const GraphDev& G, const csaw_graph_view& hv, int kind, const std::vector<int64_t>& pool,
+                  uint64_t* out, hipStream_t stream, int num_cu);
+int rmat_generate_device(int scale, int edge_factor, double a, double b, double c, uint64_t key, hipStream_t stream,
+                         int num_cu, int64_t* V, int64_t* E, int64_t** d_off, int32_t** d_adj);
 int graph_alloc_common(csaw_graph* g);
 int device_init(int* num_cu);
 }  // namespace csaw

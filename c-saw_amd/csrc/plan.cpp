@@ -716,6 +716,20 @@ extern "C" int csaw_edges_copy(const csaw_edges* e, int32_t* src, int32_t* dst, 
 
 extern "C" void csaw_edges_free(csaw_edges* e) { delete e; }
 
+extern "C" int csaw_edges_from_host(int32_t depth, int32_t samples, int64_t n, const int32_t* src, const int32_t* dst,
+                                    const int32_t* smp, const int64_t* hop_off, csaw_edges** out) {
+  if (!out || n < 0 || depth < 0 || samples < 0 || (n > 0 && (!src || !dst || !smp))) return CSAW_ERR_INVALID_ARG;
+  csaw_edges* e = new csaw_edges();
+  e->depth = depth;
+  e->samples = samples;
+  e->src.assign(src, src + n);
+  e->dst.assign(dst, dst + n);
+  e->smp.assign(smp, smp + n);
+  if (hop_off) e->hop_off.assign(hop_off, hop_off + (size_t)depth * (samples + 1));
+  *out = e;
+  return CSAW_OK;
+}
+
 // C4 request mix (all draws from Philox with the mix key, domain kDomMix).
 extern "C" int64_t csaw_request_mix(uint64_t mix_key, int32_t n, int32_t lo, int32_t hi, csaw_request* out) {
   if (n < 0 || lo < 1 || hi < lo || (n > 0 && !out)) return -1;

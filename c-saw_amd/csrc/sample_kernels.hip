@@ -259,11 +259,43 @@ __global__ __launch_bounds__(256) void k_expand_plain(GraphDev G, csaw_graph_vie
   const int64_t n = *H.n_ptr;
   const int wid = threadIdx.x / kWave;
   const int64_t nw = (int64_t)gridDim.x * 4;
-  for (int64_t e0 = (blockIdx.x * 4 + wid) * (int64_t)kWave; e0 < n; e0 += nw * kWave) {
-    const int cnt = (int)min((int64_t)kWave, n - e0);
+  // batch size adapts to the frontier: one entry per wave when it is small
+  // (all waves busy), up to 64 entries per metadata round when it is large
+  const int64_t bs = min((int64_t)kWave, max((int64_t)1, (n + nw - 1) / nw));
+  for (int64_t e0 = (blockIdx.x * 4 + wid) * bs; e0 < n; e0 += nw * bs) {
+    const int cnt = (int)min(bs, n - e0);
     wave_expand_batch<KIND>(G, hv, R, gs2req, H, nullptr, e0, cnt, KIND == kInt ? cdf + wid * kWaveIntCap : nullptr,
                             bms + wid * (kWaveBitmapCap / 32));
   }
+}
+
+// one wave per 64 listed entries (the streaming path: entries of one partition)
+template <int KIND>
+__global__ __launch_bounds__(256) void k_expand_list(GraphDev G, csaw_graph_view hv, const ReqDev* __restrict__ R,
+                                                     const int32_t* __restrict__ gs2req, HopDev H,
+                                                     const int32_t* __restrict__ list, int64_t n) {
+  __shared__ uint64_t cdf[KIND == kInt ? 4 * kWaveIntCap : 1];
+  __shared__ uint32_t bms[4 * (kWaveBitmapCap / 32)];
+  const int wid = threadIdx.x / kWave;
+  const int64_t nw = (int64_t)gridDim.x * 4;
+  const int64_t bs = min((int64_t)kWave, max((int64_t)1, (n + nw - 1) / nw));
+  for (int64_t e0 = (blockIdx.x * 4 + wid) * bs; e0 < n; e0 += nw * bs) {
+    const int cnt = (int)min(bs, n - e0);
+    wave_expand_batch<KIND>(G, hv, R, gs2req, H, list, e0, cnt, KIND == kInt ? cdf + wid * kWaveIntCap : nullptr,
+                            bms + wid * (kWaveBitmapCap / 32));
+  }
+}
+
+int launch_expand_list(const LaunchCfg& L, int kind, const GraphDev& G, const csaw_graph_view& hv, const ReqDev* R,
+                       const int32_t* gs2req, const HopDev& H, const int32_t* list, int64_t n) {
+  if (n <= 0) return CSAW_OK;
+  const int64_t nwg = (n + 4 * kWave - 1) / (4 * kWave);
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(nwg, (int64_t)L.num_cu * L.expand_per_cu));
+  if (kind == kUniform) k_expand_list<kUniform><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, list, n);
+  else if (kind == kInt) k_expand_list<kInt><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, list, n);
+  else k_expand_list<kF32><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, list, n);
+  CSAW_LAUNCH_CHECK();
+  return CSAW_OK;
 }
 
 // ------------------------------------------------------------- choose

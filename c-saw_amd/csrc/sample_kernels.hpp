@@ -86,6 +86,8 @@ int launch_digest(hipStream_t st, int num_cu, const int32_t* src, const int32_t*
 int launch_expand_plain(const LaunchCfg& L, int kind, const GraphDev& G, const csaw_graph_view& hv, const ReqDev* R,
                         const int32_t* gs2req, const HopDev& H, int64_t max_entries);
 int launch_choose(const LaunchCfg& L, const ChooseArgs& A, const HopDev& H, int64_t max_chunks);
+int launch_expand_list(const LaunchCfg& L, int kind, const GraphDev& G, const csaw_graph_view& hv, const ReqDev* R,
+                       const int32_t* gs2req, const HopDev& H, const int32_t* list, int64_t n);
 int launch_expand_chunks(const LaunchCfg& L, const GraphDev& G, const csaw_graph_view& hv, const ReqDev* R,
                          const int32_t* gs2req, const HopDev& H, const ExpandArgs& A, const bool* kinds_present,
                          TimingHook* th);

@@ -1,0 +1,487 @@
+// Streaming sampler for graphs larger than the device budget: the reference's
+// streaming path (streaming/streaming_sampling.cu:412-425 block_augument,
+// 427-728 Sampler with the per-partition queues of check(), 199-384),
+// re-designed for a real out-of-core run:
+//  - the CSR stays in pinned host memory; the offsets and degrees are
+//    device-resident (every kernel needs d(v); degree bias needs d(u) of any
+//    neighbour);
+//  - vertex ranges with equal edge counts form the partitions; per hop the
+//    frontier is binned by partition and each partition's adjacency slice is
+//    copied on a second stream into one of two device buffers while the
+//    previous partition expands (double buffering);
+//  - integer-bias hubs of the partition's frontier get their CDF built once
+//    per (hop, partition) into a pool, split across workgroups;
+//  - output positions come from the same global prefix sums as the
+//    in-memory run, so the edge list is bit-identical to it.
+// Every device allocation is charged against budget_bytes.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "graph.hpp"
+#include "sample_kernels.hpp"
+
+struct csaw_host_graph {
+  int64_t V = 0, E = 0, max_deg = 0;
+  int64_t* off = nullptr;  // pinned host, V+1
+  int32_t* adj = nullptr;  // pinned host, E
+};
+
+namespace csaw {
+
+__global__ void k_stream_degree(const int64_t* __restrict__ off, int64_t V, int32_t* __restrict__ deg) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x)
+    deg[v] = (int32_t)(off[v + 1] - off[v]);
+}
+
+// partition id per frontier entry, per-partition counts, hub marks
+__global__ void k_part_bin(GraphDev G, const ReqDev* __restrict__ R, HopDev H, const int32_t* __restrict__ vb,
+                           int P, uint8_t* __restrict__ pid, uint32_t* __restrict__ cnt,
+                           uint32_t* __restrict__ hubbits) {
+  const int64_t n = *H.n_ptr;
+  const ReqDev& r = R[0];
+  const int32_t F = r.fanout[H.hop];
+  const bool int_bias = r.bias == CSAW_BIAS_DEGREE || r.bias == CSAW_BIAS_USER;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t v = H.fv[e];
+    int lo = 0, hi = P;  // last p with vb[p] <= v
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (vb[mid] <= v) lo = mid; else hi = mid;
+    }
+    pid[e] = (uint8_t)lo;
+    atomicAdd(&cnt[lo], 1u);
+    const int32_t d = G.deg[v];
+    if (int_bias && d > kHubMin && d > F) atomicOr(&hubbits[v >> 5], 1u << (v & 31));
+  }
+}
+
+__global__ void k_part_scatter(HopDev H, const uint8_t* __restrict__ pid, const uint32_t* __restrict__ part_off,
+                               uint32_t* __restrict__ cursor, int32_t* __restrict__ list) {
+  const int64_t n = *H.n_ptr;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int p = pid[e];
+    list[part_off[p] + atomicAdd(&cursor[p], 1u)] = (int32_t)e;
+  }
+}
+
+struct Budget {
+  int64_t limit, used = 0, peak = 0;
+  std::vector<void*> ptrs;
+  explicit Budget(int64_t l) : limit(l) {}
+  template <typename T>
+  int alloc(T** p, int64_t count) {
+    const int64_t bytes = std::max<int64_t>(count, 1) * (int64_t)sizeof(T);
+    if (used + bytes > limit) {
+      set_last_error("device budget exceeded: need " + std::to_string(used + bytes) + " of " + std::to_string(limit));
+      return CSAW_ERR_BUDGET_TOO_SMALL;
+    }
+    CSAW_HIP(hipMalloc((void**)p, bytes));
+    ptrs.push_back((void*)*p);
+    used += bytes;
+    peak = std::max(peak, used);
+    return CSAW_OK;
+  }
+  ~Budget() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+};
+
+static int grid_of(int64_t n, int num_cu) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)num_cu * 8));
+}
+
+}  // namespace csaw
+
+using namespace csaw;
+
+extern "C" int csaw_host_graph_create_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t key,
+                                           csaw_host_graph** out) {
+  if (!out || scale < 1 || scale > 30 || edge_factor < 1) return CSAW_ERR_INVALID_ARG;
+  *out = nullptr;
+  int num_cu = 0;
+  CSAW_TRY(device_init(&num_cu));
+  hipStream_t st;
+  CSAW_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  int64_t V = 0, E = 0;
+  int64_t* d_off = nullptr;
+  int32_t* d_adj = nullptr;
+  int rc = rmat_generate_device(scale, edge_factor, a, b, c, key, st, num_cu, &V, &E, &d_off, &d_adj);
+  csaw_host_graph* g = nullptr;
+  if (rc == CSAW_OK) {
+    g = new csaw_host_graph();
+    g->V = V;
+    g->E = E;
+    auto body = [&]() -> int {
+      CSAW_HIP(hipHostMalloc((void**)&g->off, sizeof(int64_t) * (V + 1), hipHostMallocDefault));
+      CSAW_HIP(hipHostMalloc((void**)&g->adj, sizeof(int32_t) * std::max<int64_t>(E, 1), hipHostMallocDefault));
+      CSAW_HIP(hipMemcpy(g->off, d_off, sizeof(int64_t) * (V + 1), hipMemcpyDeviceToHost));
+      if (E > 0) CSAW_HIP(hipMemcpy(g->adj, d_adj, sizeof(int32_t) * E, hipMemcpyDeviceToHost));
+      return CSAW_OK;
+    };
+    rc = body();
+  }
+  (void)hipFree(d_off);
+  (void)hipFree(d_adj);
+  (void)hipStreamDestroy(st);
+  if (rc != CSAW_OK) {
+    if (g) {
+      (void)hipHostFree(g->off);
+      (void)hipHostFree(g->adj);
+      delete g;
+    }
+    return rc;
+  }
+  for (int64_t v = 0; v < V; ++v) g->max_deg = std::max(g->max_deg, g->off[v + 1] - g->off[v]);
+  *out = g;
+  return CSAW_OK;
+}
+
+extern "C" int csaw_host_graph_from_csr(const int64_t* off, const int32_t* adj, int64_t V, int64_t E,
+                                        csaw_host_graph** out) {
+  if (!off || (E > 0 && !adj) || !out || V < 0 || V > 0x7fffffffLL || off[0] != 0 || off[V] != E)
+    return CSAW_ERR_INVALID_ARG;
+  for (int64_t v = 0; v < V; ++v)
+    if (off[v + 1] < off[v]) return CSAW_ERR_INVALID_GRAPH;
+  for (int64_t i = 0; i < E; ++i)
+    if (adj[i] < 0 || adj[i] >= V) return CSAW_ERR_INVALID_GRAPH;
+  csaw_host_graph* g = new csaw_host_graph();
+  g->V = V;
+  g->E = E;
+  if (hipHostMalloc((void**)&g->off, sizeof(int64_t) * (V + 1), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&g->adj, sizeof(int32_t) * std::max<int64_t>(E, 1), hipHostMallocDefault) != hipSuccess) {
+    (void)hipHostFree(g->off);
+    delete g;
+    return CSAW_ERR_OOM;
+  }
+  memcpy(g->off, off, sizeof(int64_t) * (V + 1));
+  if (E > 0) memcpy(g->adj, adj, sizeof(int32_t) * E);
+  for (int64_t v = 0; v < V; ++v) g->max_deg = std::max(g->max_deg, off[v + 1] - off[v]);
+  *out = g;
+  return CSAW_OK;
+}
+
+extern "C" int csaw_host_graph_info(const csaw_host_graph* g, int64_t* V, int64_t* E, int64_t* max_degree,
+                                    const int64_t** offsets, const int32_t** indices) {
+  if (!g) return CSAW_ERR_INVALID_ARG;
+  if (V) *V = g->V;
+  if (E) *E = g->E;
+  if (max_degree) *max_degree = g->max_deg;
+  if (offsets) *offsets = g->off;
+  if (indices) *indices = g->adj;
+  return CSAW_OK;
+}
+
+extern "C" int csaw_host_graph_destroy(csaw_host_graph* g) {
+  if (!g) return CSAW_OK;
+  (void)hipHostFree(g->off);
+  (void)hipHostFree(g->adj);
+  delete g;
+  return CSAW_OK;
+}
+
+extern "C" int csaw_stream_sample(csaw_host_graph* hg, const csaw_request* req, int64_t budget_bytes,
+                                  int32_t n_partitions, csaw_edges** out, csaw_stream_stats* stats) {
+  if (!hg || !req || !out || n_partitions < 1 || n_partitions > 255 || budget_bytes <= 0) return CSAW_ERR_INVALID_ARG;
+  *out = nullptr;
+  const csaw_request& r = *req;
+  if (r.samples < 0 || r.depth < 0 || r.depth > CSAW_MAX_HOPS || (r.samples > 0 && !r.seeds)) return CSAW_ERR_INVALID_ARG;
+  if (r.bias == CSAW_BIAS_WEIGHT) {
+    set_last_error("streaming supports uniform, degree and user biases");
+    return CSAW_ERR_UNSUPPORTED;
+  }
+  for (int h = 0; h < r.depth; ++h)
+    if (r.fanouts[h] < 0 || r.fanouts[h] > CSAW_MAX_FANOUT) return CSAW_ERR_INVALID_ARG;
+  for (int32_t s = 0; s < r.samples; ++s)
+    if (r.seeds[s] < 0 || r.seeds[s] >= hg->V) return CSAW_ERR_INVALID_ARG;
+  int num_cu = 0;
+  CSAW_TRY(device_init(&num_cu));
+  const int64_t V = hg->V, E = hg->E;
+  const int P = n_partitions;
+  const int32_t S = r.samples;
+  // equal-edge vertex ranges
+  std::vector<int32_t> vb(P + 1);
+  vb[0] = 0;
+  vb[P] = (int32_t)V;
+  for (int p = 1; p < P; ++p) {
+    const int64_t target = E * p / P;
+    vb[p] = (int32_t)(std::lower_bound(hg->off, hg->off + V + 1, target) - hg->off);
+    vb[p] = std::max(vb[p], vb[p - 1]);
+  }
+  int64_t slice_max = 1, hubs_max = 1;
+  for (int p = 0; p < P; ++p) {
+    slice_max = std::max(slice_max, hg->off[vb[p + 1]] - hg->off[vb[p]]);
+    int64_t nh = 0;
+    for (int32_t v = vb[p]; v < vb[p + 1]; ++v) nh += (hg->off[v + 1] - hg->off[v]) > kHubMin;
+    hubs_max = std::max(hubs_max, nh);
+  }
+  // per-hop capacities
+  std::vector<int64_t> cap(r.depth), nmax(r.depth);
+  int64_t ub = S, mx = std::max<int32_t>(S, 1);
+  for (int h = 0; h < r.depth; ++h) {
+    nmax[h] = ub;
+    ub = ub * r.fanouts[h];
+    cap[h] = ub;
+    mx = std::max(mx, nmax[h]);
+    if (ub > ((int64_t)1 << 32)) return CSAW_ERR_CAPACITY;
+  }
+  const bool int_bias = r.bias == CSAW_BIAS_DEGREE || r.bias == CSAW_BIAS_USER;
+  const int kind = bias_kind(r.bias);
+
+  Budget B(budget_bytes);
+  hipStream_t cs = nullptr, xs = nullptr;
+  hipEvent_t ready[2] = {nullptr, nullptr}, freed[2] = {nullptr, nullptr}, t0 = nullptr, t1 = nullptr;
+  std::vector<int32_t> h_src, h_dst, h_smp;
+  std::vector<int64_t> h_ho;
+  double h2d_bytes = 0, h2d_ms = 0;
+  auto body = [&]() -> int {
+    CSAW_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    CSAW_HIP(hipStreamCreateWithFlags(&xs, hipStreamNonBlocking));
+    for (int i = 0; i < 2; ++i) {
+      CSAW_HIP(hipEventCreateWithFlags(&ready[i], hipEventDisableTiming));
+      CSAW_HIP(hipEventCreateWithFlags(&freed[i], hipEventDisableTiming));
+    }
+    CSAW_HIP(hipEventCreate(&t0));
+    CSAW_HIP(hipEventCreate(&t1));
+    // resident: offsets, degrees
+    int64_t* d_off;
+    int32_t* d_deg;
+    CSAW_TRY(B.alloc(&d_off, V + 1));
+    CSAW_TRY(B.alloc(&d_deg, V));
+    CSAW_HIP(hipMemcpyAsync(d_off, hg->off, sizeof(int64_t) * (V + 1), hipMemcpyHostToDevice, cs));
+    k_stream_degree<<<grid_of(V, num_cu), 256, 0, cs>>>(d_off, V, d_deg);
+    CSAW_LAUNCH_CHECK();
+    int32_t* slice[2];
+    CSAW_TRY(B.alloc(&slice[0], slice_max));
+    CSAW_TRY(B.alloc(&slice[1], slice_max));
+    uint64_t* pool = nullptr;
+    int32_t* d_hub_vid = nullptr;
+    int64_t* d_hub_pool = nullptr;
+    uint32_t* d_hubbits = nullptr;
+    if (int_bias) {
+      CSAW_TRY(B.alloc(&pool, slice_max));
+      CSAW_TRY(B.alloc(&d_hubbits, (V + 31) / 32));
+      CSAW_TRY(B.alloc(&d_hub_vid, hubs_max));
+      CSAW_TRY(B.alloc(&d_hub_pool, hubs_max + 1));
+    }
+    // request / frontier / outputs
+    ReqDev rd;
+    memset(&rd, 0, sizeof(rd));
+    for (int h = 0; h < r.depth; ++h) rd.fanout[h] = r.fanouts[h];
+    rd.depth = r.depth;
+    rd.bias = r.bias;
+    rd.dedup = r.dedup ? 1 : 0;
+    rd.samples = S;
+    rd.key_lo = (uint32_t)r.seed;
+    rd.key_hi = (uint32_t)(r.seed >> 32);
+    rd.request_id = r.request_id;
+    ReqDev* d_req;
+    int32_t *d_gs2req, *d_fv0, *d_fs0;
+    int64_t* d_fo0;
+    CSAW_TRY(B.alloc(&d_req, 1));
+    CSAW_TRY(B.alloc(&d_gs2req, S));
+    CSAW_TRY(B.alloc(&d_fv0, S));
+    CSAW_TRY(B.alloc(&d_fs0, S));
+    CSAW_TRY(B.alloc(&d_fo0, S + 1));
+    std::vector<int32_t> zeros(S, 0), iota_s(S);
+    std::vector<int64_t> fo0(S + 1);
+    std::iota(iota_s.begin(), iota_s.end(), 0);
+    std::iota(fo0.begin(), fo0.end(), (int64_t)0);
+    CSAW_HIP(hipMemcpyAsync(d_req, &rd, sizeof(rd), hipMemcpyHostToDevice, cs));
+    if (S > 0) {
+      CSAW_HIP(hipMemcpyAsync(d_gs2req, zeros.data(), sizeof(int32_t) * S, hipMemcpyHostToDevice, cs));
+      CSAW_HIP(hipMemcpyAsync(d_fv0, r.seeds, sizeof(int32_t) * S, hipMemcpyHostToDevice, cs));
+      CSAW_HIP(hipMemcpyAsync(d_fs0, iota_s.data(), sizeof(int32_t) * S, hipMemcpyHostToDevice, cs));
+    }
+    CSAW_HIP(hipMemcpyAsync(d_fo0, fo0.data(), sizeof(int64_t) * (S + 1), hipMemcpyHostToDevice, cs));
+    std::vector<int64_t*> d_eo(r.depth);
+    std::vector<int32_t*> d_src(r.depth), d_dst(r.depth), d_smp(r.depth);
+    for (int h = 0; h < r.depth; ++h) {
+      CSAW_TRY(B.alloc(&d_eo[h], S + 1));
+      CSAW_TRY(B.alloc(&d_src[h], cap[h]));
+      CSAW_TRY(B.alloc(&d_dst[h], cap[h]));
+      CSAW_TRY(B.alloc(&d_smp[h], cap[h]));
+    }
+    const int64_t nblk = (mx + kBlockEntries - 1) / kBlockEntries;
+    uint32_t *d_local, *d_blk, *d_cnt, *d_poff, *d_cursor;
+    uint64_t* d_grp;
+    int64_t* d_tot;
+    uint8_t* d_pid;
+    int32_t *d_list, *d_vb;
+    CSAW_TRY(B.alloc(&d_local, mx));
+    CSAW_TRY(B.alloc(&d_blk, nblk));
+    CSAW_TRY(B.alloc(&d_grp, (nblk + 1023) / 1024));
+    CSAW_TRY(B.alloc(&d_tot, std::max(r.depth, 1)));
+    CSAW_TRY(B.alloc(&d_pid, mx));
+    CSAW_TRY(B.alloc(&d_list, mx));
+    CSAW_TRY(B.alloc(&d_cnt, 3 * P));
+    d_poff = d_cnt + P;
+    d_cursor = d_cnt + 2 * P;
+    CSAW_TRY(B.alloc(&d_vb, P + 1));
+    CSAW_HIP(hipMemcpyAsync(d_vb, vb.data(), sizeof(int32_t) * (P + 1), hipMemcpyHostToDevice, cs));
+
+    GraphDev G;
+    memset(&G, 0, sizeof(G));
+    G.off = d_off;
+    G.deg = d_deg;
+    G.V = V;
+    G.E = E;
+    csaw_graph_view hv;
+    hv.beg_pos = d_off;
+    hv.adj_list = nullptr;
+    hv.degree_list = d_deg;
+    hv.weight_list = nullptr;
+    hv.vert_count = V;
+    hv.edge_count = E;
+    LaunchCfg L;
+    L.stream = cs;
+    L.num_cu = num_cu;
+    std::vector<uint32_t> hubbits_h;
+    CSAW_HIP(hipEventRecord(t0, cs));
+    for (int h = 0; h < r.depth; ++h) {
+      HopDev H;
+      memset(&H, 0, sizeof(H));
+      H.fv = h == 0 ? d_fv0 : d_dst[h - 1];
+      H.fs = h == 0 ? d_fs0 : d_smp[h - 1];
+      H.fo = h == 0 ? d_fo0 : d_eo[h - 1];
+      H.n_ptr = H.fo + S;
+      H.hop = h;
+      H.gb = S;
+      H.local_excl = d_local;
+      H.blk_excl = d_blk;
+      H.grp_excl = d_grp;
+      H.src = d_src[h];
+      H.dst = d_dst[h];
+      H.smp = d_smp[h];
+      H.eo = d_eo[h];
+      CSAW_TRY(launch_count(L, G, d_req, d_gs2req, H, false, nmax[h]));
+      CSAW_TRY(launch_scans(L, H, d_tot + h, nmax[h]));
+      // bin the frontier by partition
+      CSAW_TRY(launch_fill(L, d_cnt, 0u, 3 * P));
+      if (int_bias) CSAW_TRY(launch_fill(L, d_hubbits, 0u, (V + 31) / 32));
+      k_part_bin<<<grid_of(nmax[h], num_cu), 256, 0, cs>>>(G, d_req, H, d_vb, P, d_pid, d_cnt, d_hubbits);
+      CSAW_LAUNCH_CHECK();
+      std::vector<uint32_t> cnt(P), poff(P + 1, 0);
+      CSAW_HIP(hipMemcpyAsync(cnt.data(), d_cnt, sizeof(uint32_t) * P, hipMemcpyDeviceToHost, cs));
+      CSAW_HIP(hipStreamSynchronize(cs));
+      for (int p = 0; p < P; ++p) poff[p + 1] = poff[p] + cnt[p];
+      CSAW_HIP(hipMemcpyAsync(d_poff, poff.data(), sizeof(uint32_t) * P, hipMemcpyHostToDevice, cs));
+      k_part_scatter<<<grid_of(nmax[h], num_cu), 256, 0, cs>>>(H, d_pid, d_poff, d_cursor, d_list);
+      CSAW_LAUNCH_CHECK();
+      // unique frontier hubs per partition (sorted by id)
+      std::vector<std::vector<int32_t>> hubs(P);
+      if (int_bias) {
+        hubbits_h.resize((V + 31) / 32);
+        CSAW_HIP(hipMemcpyAsync(hubbits_h.data(), d_hubbits, sizeof(uint32_t) * hubbits_h.size(),
+                                hipMemcpyDeviceToHost, cs));
+        CSAW_HIP(hipStreamSynchronize(cs));
+        int p = 0;
+        for (size_t wi = 0; wi < hubbits_h.size(); ++wi) {
+          uint32_t m = hubbits_h[wi];
+          while (m) {
+            const int32_t v = (int32_t)(wi * 32 + __builtin_ctz(m));
+            m &= m - 1;
+            while (v >= vb[p + 1]) ++p;
+            hubs[p].push_back(v);
+          }
+        }
+      }
+      // partitions in order, slice copies double-buffered on the copy stream
+      auto issue_copy = [&](int p, int buf) -> int {
+        const int64_t a = hg->off[vb[p]], b = hg->off[vb[p + 1]];
+        CSAW_HIP(hipStreamWaitEvent(xs, freed[buf], 0));
+        if (b > a) CSAW_HIP(hipMemcpyAsync(slice[buf], hg->adj + a, sizeof(int32_t) * (b - a), hipMemcpyHostToDevice, xs));
+        CSAW_HIP(hipEventRecord(ready[buf], xs));
+        h2d_bytes += (double)(b - a) * sizeof(int32_t);
+        return CSAW_OK;
+      };
+      std::vector<int> todo;
+      for (int p = 0; p < P; ++p)
+        if (cnt[p] > 0) todo.push_back(p);
+      // the freed events start signalled
+      for (int i = 0; i < 2; ++i) CSAW_HIP(hipEventRecord(freed[i], cs));
+      if (!todo.empty()) CSAW_TRY(issue_copy(todo[0], 0));
+      for (size_t t = 0; t < todo.size(); ++t) {
+        const int p = todo[t], buf = (int)(t & 1);
+        if (t + 1 < todo.size()) CSAW_TRY(issue_copy(todo[t + 1], (int)((t + 1) & 1)));
+        CSAW_HIP(hipStreamWaitEvent(cs, ready[buf], 0));
+        GraphDev Gp = G;
+        Gp.adj = slice[buf] - hg->off[vb[p]];  // global offsets index the slice
+        csaw_graph_view hvp = hv;
+        hvp.adj_list = Gp.adj;
+        if (int_bias && !hubs[p].empty()) {
+          std::vector<int64_t> hp(hubs[p].size() + 1, 0);
+          for (size_t i = 0; i < hubs[p].size(); ++i)
+            hp[i + 1] = hp[i] + (hg->off[hubs[p][i] + 1] - hg->off[hubs[p][i]]);
+          CSAW_HIP(hipMemcpyAsync(d_hub_vid, hubs[p].data(), sizeof(int32_t) * hubs[p].size(), hipMemcpyHostToDevice, cs));
+          CSAW_HIP(hipMemcpyAsync(d_hub_pool, hp.data(), sizeof(int64_t) * hp.size(), hipMemcpyHostToDevice, cs));
+          Gp.hub_vid = d_hub_vid;
+          Gp.hub_pool = d_hub_pool;
+          Gp.n_hubs = (int32_t)hubs[p].size();
+          Gp.hub_cdf[0] = Gp.hub_cdf[1] = pool;
+          CSAW_TRY(hub_cdf_build(Gp, hvp, r.bias == CSAW_BIAS_USER ? 1 : 0, hp, pool, cs, num_cu));
+        } else {
+          Gp.n_hubs = 0;
+        }
+        CSAW_TRY(launch_expand_list(L, kind, Gp, hvp, d_req, d_gs2req, H, d_list + poff[p], (int64_t)cnt[p]));
+        CSAW_HIP(hipEventRecord(freed[buf], cs));
+        if (int_bias && !hubs[p].empty()) CSAW_HIP(hipStreamSynchronize(cs));  // hub list buffers are reused
+      }
+    }
+    CSAW_HIP(hipEventRecord(t1, cs));
+    CSAW_HIP(hipEventSynchronize(t1));
+    float ms = 0.f;
+    CSAW_HIP(hipEventElapsedTime(&ms, t0, t1));
+    h2d_ms = ms;
+    // copy out
+    std::vector<int64_t> lo(r.depth), hi(r.depth);
+    int64_t n_all = 0;
+    for (int h = 0; h < r.depth; ++h) {
+      CSAW_HIP(hipMemcpy(&lo[h], d_eo[h], sizeof(int64_t), hipMemcpyDeviceToHost));
+      CSAW_HIP(hipMemcpy(&hi[h], d_eo[h] + S, sizeof(int64_t), hipMemcpyDeviceToHost));
+      n_all += hi[h] - lo[h];
+    }
+    h_src.resize(n_all);
+    h_dst.resize(n_all);
+    h_smp.resize(n_all);
+    h_ho.resize((size_t)r.depth * (S + 1));
+    int64_t at = 0;
+    for (int h = 0; h < r.depth; ++h) {
+      const int64_t n = hi[h] - lo[h];
+      if (n > 0) {
+        CSAW_HIP(hipMemcpy(h_src.data() + at, d_src[h] + lo[h], sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        CSAW_HIP(hipMemcpy(h_dst.data() + at, d_dst[h] + lo[h], sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        CSAW_HIP(hipMemcpy(h_smp.data() + at, d_smp[h] + lo[h], sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+      }
+      std::vector<int64_t> eo(S + 1);
+      CSAW_HIP(hipMemcpy(eo.data(), d_eo[h], sizeof(int64_t) * (S + 1), hipMemcpyDeviceToHost));
+      for (int32_t s = 0; s <= S; ++s) h_ho[(size_t)h * (S + 1) + s] = at + eo[s] - lo[h];
+      at += n;
+    }
+    return CSAW_OK;
+  };
+  int st = body();
+  for (int i = 0; i < 2; ++i) {
+    if (ready[i]) (void)hipEventDestroy(ready[i]);
+    if (freed[i]) (void)hipEventDestroy(freed[i]);
+  }
+  if (t0) (void)hipEventDestroy(t0);
+  if (t1) (void)hipEventDestroy(t1);
+  if (cs) (void)hipStreamSynchronize(cs);
+  if (xs) (void)hipStreamSynchronize(xs);
+  if (cs) (void)hipStreamDestroy(cs);
+  if (xs) (void)hipStreamDestroy(xs);
+  if (st != CSAW_OK) return st;
+  if (stats) {
+    memset(stats, 0, sizeof(*stats));
+    stats->edges = (int64_t)h_src.size();
+    stats->device_ms = h2d_ms;
+    stats->h2d_bytes = h2d_bytes;
+    stats->partitions = P;
+    stats->peak_device_bytes = B.peak;
+    stats->slice_bytes = slice_max * (int64_t)sizeof(int32_t);
+  }
+  return csaw_edges_from_host(r.depth, S, (int64_t)h_src.size(), h_src.data(), h_dst.data(), h_smp.data(),
+                              h_ho.data(), out);
+}

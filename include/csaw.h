@@ -99,8 +99,11 @@ int csaw_graph_info(const csaw_graph *g, int64_t *V, int64_t *E, int64_t *max_de
 int csaw_graph_copy_csr(const csaw_graph *g, int64_t *offsets, int32_t *indices, float *weights);
 int csaw_graph_destroy(csaw_graph *g);
 
-/* Seeds A.8: seed_i = mulhi64(philox(key; i, request), V), written to host. */
+/* Seeds: seed_i = mulhi64(philox(key; i, 0, request, DOM_SEEDS), V), written
+ * to host (the reference drew mt19937 seeds in [1,10000], main.cu:257-262). */
 int csaw_seeds(const csaw_graph *g, uint64_t key, uint32_t request, int64_t n, int32_t *out);
+/* The same seeds computed on the host for a vertex count V. */
+int csaw_seeds_v(int64_t V, uint64_t key, uint32_t request, int64_t n, int32_t *out);
 
 /* ------------------------------------------------------------- sampling */
 /* The Sampling-class parameter surface (sample_class.cuh:146-185 and the
@@ -249,6 +252,40 @@ int csaw_plan_chunk_log(const csaw_plan *p, float *features, int32_t *variant, f
 /* Tries launch shapes on the given requests and returns the fastest. */
 int csaw_autotune(csaw_graph *g, const csaw_request *reqs, int32_t n_req, const csaw_plan_opts *base,
                   csaw_tuning *best, double *best_ms);
+
+/* --------------------------------------------- streaming (out-of-core graphs) */
+/* A CSR kept in pinned host memory (the reference's streaming sampler,
+ * streaming/streaming_sampling.cu:427-728, which copied the whole graph to
+ * the device first; here only partition slices travel). */
+typedef struct csaw_host_graph csaw_host_graph;
+/* RMAT generated on the GPU, then moved to pinned host memory. */
+int csaw_host_graph_create_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t key,
+                                csaw_host_graph **out);
+int csaw_host_graph_from_csr(const int64_t *offsets, const int32_t *indices, int64_t V, int64_t E,
+                             csaw_host_graph **out);
+/* offsets / indices: the pinned host arrays (valid until destroy). */
+int csaw_host_graph_info(const csaw_host_graph *g, int64_t *V, int64_t *E, int64_t *max_degree,
+                         const int64_t **offsets, const int32_t **indices);
+int csaw_host_graph_destroy(csaw_host_graph *g);
+
+typedef struct csaw_stream_stats {
+  int64_t edges;
+  double device_ms;          /* device time of the whole streamed run       */
+  double h2d_bytes;          /* adjacency bytes copied host -> device       */
+  int32_t partitions;
+  int64_t peak_device_bytes; /* budget charged at the peak                  */
+  int64_t slice_bytes;       /* largest partition slice                     */
+} csaw_stream_stats;
+
+/* Samples one request (uniform, degree or user bias) streaming vertex-range
+ * partitions through two device buffers under budget_bytes of device
+ * memory; CSAW_ERR_BUDGET_TOO_SMALL when the resident state does not fit.
+ * The edge list equals the in-memory csaw_sample result. */
+int csaw_stream_sample(csaw_host_graph *g, const csaw_request *req, int64_t budget_bytes,
+                       int32_t n_partitions, csaw_edges **out, csaw_stream_stats *stats);
+/* Wraps host arrays as a csaw_edges result (copies them). */
+int csaw_edges_from_host(int32_t depth, int32_t samples, int64_t n, const int32_t *src, const int32_t *dst,
+                         const int32_t *sample, const int64_t *hop_offsets, csaw_edges **out);
 
 /* C4 request mix: n requests, seed counts log-uniform in [lo, hi], fanouts
  * {25,10} or {15,10,5}, bias uniform / degree / weight with p = 1/3 each,
