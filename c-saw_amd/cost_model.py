@@ -19,8 +19,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 MODEL_DIR = os.path.join(HERE, "models")
 DEFAULT_MODEL = os.path.join(MODEL_DIR, "cost_model_gfx950.json")
-# workgroups per CU of each variant's kernel (LaunchCfg defaults)
-WG_PER_CU = {0: 4, 1: 8, 2: 8, 3: 2, 4: 2}
+# concurrent chunk workers per CU of each variant's kernel (LaunchCfg
+# defaults): THREAD and WAVE kernels claim chunks per wave (4 waves per
+# workgroup), BLOCK per workgroup
+WG_PER_CU = {0: 16, 1: 32, 2: 32, 3: 2, 4: 2}
 
 
 def _runs(C, g, specs, sched, variant=0, cdf_cache=True, explore_key=0, tuning=None):

@@ -85,7 +85,8 @@ struct Chunk {
   float sum_d;
   uint32_t key;    // block * kNumBuckets + bucket (stable across runs)
   uint8_t max_f;
-  uint8_t pad[3];
+  uint8_t pad;
+  uint16_t sum_l;  // sum of ceil(log2 d) over the chunk's entries
 };
 
 struct HopDev {
@@ -130,13 +131,27 @@ __host__ __device__ __forceinline__ int bias_kind(int32_t bias) {
   return bias == CSAW_BIAS_UNIFORM ? kUniform : (bias == CSAW_BIAS_WEIGHT ? kF32 : kInt);
 }
 
-// bucket id: 0 copy-all, else 1 + 32*kind + ceil_log2(d)  (kind 0..2, log 0..31)
+// Degree classes of a chunk (by ceil(log2 d)): <=4, <=16, <=32, <=128,
+// <=1024, <=8192, <=65536, larger.  Coarse classes keep chunks large.
+constexpr int kDegClasses = 8;
+__host__ __device__ __forceinline__ int degree_class(int l) {
+  return l <= 2 ? 0 : l <= 4 ? 1 : l <= 5 ? 2 : l <= 7 ? 3 : l <= 10 ? 4 : l <= 13 ? 5 : l <= 16 ? 6 : 7;
+}
+// upper bound of ceil(log2 d) in a class
+__host__ __device__ __forceinline__ int class_max_log(int cls) {
+  const int m[kDegClasses] = {2, 4, 5, 7, 10, 13, 16, 31};
+  return m[cls & 7];
+}
+
+// bucket id: 0 copy-all, else 1 + kDegClasses*kind + degree class (kind 0..2)
 __host__ __device__ __forceinline__ int entry_bucket(int64_t d, int32_t F, int32_t dedup, int32_t bias) {
   if (dedup && d <= F) return kBucketCopy;
   int kind = bias_kind(bias);
   int l = ceil_log2_u32((uint32_t)(d > 0x7fffffff ? 0x7fffffff : d));
-  return 1 + 32 * kind + l;
+  return 1 + kDegClasses * kind + degree_class(l);
 }
+__host__ __device__ __forceinline__ int bucket_kind(int bucket) { return (bucket - 1) / kDegClasses; }
+__host__ __device__ __forceinline__ int bucket_class(int bucket) { return (bucket - 1) % kDegClasses; }
 
 // Algorithmic bytes of one expansion (each datum counted once per use, no
 // cache-line amplification): frontier entry 8 + offset 8 + degree 4 = 20;

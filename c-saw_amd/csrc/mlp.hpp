@@ -66,7 +66,27 @@ __host__ __device__ __forceinline__ void chunk_features(uint32_t len, float sum_
   f[12] = max_d <= 1024 ? 1.0f : 0.0f;
   f[13] = max_d <= 8192 ? 1.0f : 0.0f;
   f[14] = flog2p1((float)n_chunks) / 24.0f;
-  f[15] = (float)(bucket & 31) / 32.0f;
+  f[15] = (float)((bucket - 1) % 8) / 8.0f;  // degree class
+}
+
+// Forward pass over weights already widened to FP32 (e.g. staged in LDS).
+__host__ __device__ __forceinline__ void mlp_forward1_f32(const float* __restrict__ w, const float* x, float* out) {
+  float h1[kMlpHid], h2[kMlpHid];
+  for (int j = 0; j < kMlpHid; ++j) {
+    float s = w[kMlpB1 + j];
+    for (int i = 0; i < kMlpIn; ++i) s += w[kMlpW1 + j * kMlpIn + i] * x[i];
+    h1[j] = s > 0.0f ? s : 0.0f;
+  }
+  for (int j = 0; j < kMlpHid; ++j) {
+    float s = w[kMlpB2 + j];
+    for (int i = 0; i < kMlpHid; ++i) s += w[kMlpW2 + j * kMlpHid + i] * h1[i];
+    h2[j] = s > 0.0f ? s : 0.0f;
+  }
+  for (int j = 0; j < kMlpOut; ++j) {
+    float s = w[kMlpB3 + j];
+    for (int i = 0; i < kMlpHid; ++i) s += w[kMlpW3 + j * kMlpHid + i] * h2[i];
+    out[j] = s;
+  }
 }
 
 // Forward pass for one feature vector (FP32 accumulate over BF16 weights).

@@ -32,6 +32,7 @@ __global__ __launch_bounds__(256) void k_count(GraphDev G, const ReqDev* __restr
   __shared__ uint32_t fmax[kNumBuckets];
   __shared__ uint32_t dmax[kNumBuckets];
   __shared__ unsigned long long dsum[kNumBuckets];
+  __shared__ uint32_t lsum[kNumBuckets];
   __shared__ uint32_t nz_cnt[2];
   __shared__ uint32_t chunk_base;
   const int64_t n = *H.n_ptr;
@@ -41,7 +42,7 @@ __global__ __launch_bounds__(256) void k_count(GraphDev G, const ReqDev* __restr
     const int64_t base = blk << kBlockShift;
     if (BIN) {
       for (int b = tid; b < kNumBuckets; b += 256) {
-        hist[b] = 0; fsum[b] = 0; fmax[b] = 0; dmax[b] = 0; dsum[b] = 0;
+        hist[b] = 0; fsum[b] = 0; fmax[b] = 0; dmax[b] = 0; dsum[b] = 0; lsum[b] = 0;
       }
       __syncthreads();
     }
@@ -75,6 +76,7 @@ __global__ __launch_bounds__(256) void k_count(GraphDev G, const ReqDev* __restr
           atomicMax(&fmax[bk[i]], (uint32_t)F);
           atomicMax(&dmax[bk[i]], (uint32_t)d);
           atomicAdd(&dsum[bk[i]], (unsigned long long)d);
+          atomicAdd(&lsum[bk[i]], (uint32_t)ceil_log2_u32((uint32_t)d));
         }
       }
       tsum += k[i];
@@ -132,6 +134,8 @@ __global__ __launch_bounds__(256) void k_count(GraphDev G, const ReqDev* __restr
           c.sum_d = (float)dsum[tid];
           c.key = (uint32_t)(blk * kNumBuckets + tid);
           c.max_f = (uint8_t)fmax[tid];
+          c.pad = 0;
+          c.sum_l = (uint16_t)min(lsum[tid], 65535u);
           H.chunks[ci] = c;
         } else {
           atomicOr(H.flags, 1u);
@@ -318,11 +322,12 @@ __device__ int heuristic_variant(int kind, const Chunk& c, bool cache) {
   return CSAW_VAR_BLOCK_BITMAP;
 }
 
-// Learned choice: argmin of the predicted log2 time over the valid variants.
-__device__ int mlp_choose(const ChooseArgs& A, const Chunk& c, int kind, const HopDev& H) {
+// Learned choice: argmin of the predicted log2 time over the valid variants
+// (wl: the BF16 weights widened to FP32 in LDS).
+__device__ int mlp_choose(const ChooseArgs& A, const float* wl, const Chunk& c, int kind, const HopDev& H) {
   float f[kMlpIn], o[kMlpOut];
   chunk_features(c.len, c.sum_d, c.max_d, c.sum_f, c.max_f, kind, c.hop, c.bucket, *H.n_chunks, f);
-  mlp_forward1(A.mlp.w, f, o);
+  mlp_forward1_f32(wl, f, o);
   int best = -1;
   float bv = 3.0e38f;
   for (int v = 0; v < kMlpOut; ++v)
@@ -333,7 +338,13 @@ __device__ int mlp_choose(const ChooseArgs& A, const Chunk& c, int kind, const H
   return best >= 0 ? best : heuristic_variant(kind, c, A.int_cache);
 }
 
-__global__ void k_choose(ChooseArgs A, HopDev H) {
+__global__ __launch_bounds__(256) void k_choose(ChooseArgs A, HopDev H) {
+  __shared__ float wl[kMlpParams];
+  const bool use_mlp = A.sched == CSAW_SCHED_MLP && A.mlp_w;
+  if (use_mlp) {
+    for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) wl[i] = bf16_to_f32(A.mlp.w[i]);
+    __syncthreads();
+  }
   const uint32_t nc = min(*H.n_chunks, H.chunk_cap);
   for (uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x; ci - threadIdx.x < nc; ci += gridDim.x * blockDim.x) {
     const bool act = ci < nc;
@@ -344,7 +355,7 @@ __global__ void k_choose(ChooseArgs A, HopDev H) {
         q = kQueueCopy;
         if (H.bytes) atomicAdd(&H.bytes[kKernCopy], (unsigned long long)expansion_bytes(0, false, c.len, (int64_t)c.sum_d, 0, 0, true));
       } else {
-        const int kind = (c.bucket - 1) >> 5;
+        const int kind = bucket_kind(c.bucket);
         int var = heuristic_variant(kind, c, A.int_cache);
         if (A.sched == CSAW_SCHED_FORCED) {
           if (variant_valid(A.variant, kind, c, A.int_cache)) var = A.variant;
@@ -355,16 +366,17 @@ __global__ void k_choose(ChooseArgs A, HopDev H) {
           for (int t = 0; t < CSAW_NUM_VARIANTS && !variant_valid(cand, kind, c, A.int_cache); ++t)
             cand = (cand + 1) % CSAW_NUM_VARIANTS;
           var = cand;
-        } else if (A.sched == CSAW_SCHED_MLP && A.mlp_w) {
-          var = mlp_choose(A, c, kind, H);
+        } else if (use_mlp) {
+          var = mlp_choose(A, wl, c, kind, H);
         }
         q = var * 3 + kind;
         if (A.chosen) A.chosen[ci] = (uint8_t)var;
         if (H.bytes) {
-          const int l = (c.bucket - 1) & 31;
+          // mean probes per draw: the chunk's mean ceil(log2 d)
+          const int l = (c.sum_l + c.len / 2) / max((int)c.len, 1);
           const int kk = var == CSAW_VAR_THREAD ? kKernThread : (var <= CSAW_VAR_WAVE_HASH ? kKernWave : kKernBlock);
           const int cap_l = var == CSAW_VAR_THREAD ? 5 : (var <= CSAW_VAR_WAVE_HASH ? 10 : 13);
-          const bool built = kind == kInt && !A.int_cache && l <= cap_l;
+          const bool built = kind == kInt && !A.int_cache && class_max_log(bucket_class(c.bucket)) <= cap_l;
           atomicAdd(&H.bytes[kk], (unsigned long long)expansion_bytes(kind, built, c.len, (int64_t)c.sum_d, c.sum_f, l, false));
         }
       }
@@ -403,6 +415,33 @@ __device__ __forceinline__ ChunkCursor next_chunk(const ExpandArgs& A, int q, ui
   return c;
 }
 
+// Wave-level work queue: lane 0 claims the next chunk of queue q; every lane
+// gets the same answer (no workgroup barrier).
+__device__ __forceinline__ ChunkCursor next_chunk_wave(const ExpandArgs& A, int q) {
+  uint32_t ci = 0xFFFFFFFFu;
+  if (lane_id() == 0) {
+    const uint32_t i = atomicAdd(&A.qhead[q], 1u);
+    if (i < A.qlen[q]) ci = A.queues[(int64_t)q * A.queue_cap + i];
+  }
+  ChunkCursor c;
+  c.ci = readlane_u(ci, 0);
+  c.valid = c.ci != 0xFFFFFFFFu;
+  return c;
+}
+
+__device__ __forceinline__ void log_chunk_wave(const ExpandArgs& A, const HopDev& H, uint32_t ci, int var,
+                                               uint64_t t0) {
+  if (A.log && lane_id() == 0) {
+    ChunkLog L;
+    L.key = H.chunks[ci].key;
+    L.hop = (uint8_t)H.hop;
+    L.variant = (uint8_t)var;
+    L.ticks = (uint32_t)(wall_clock64() - t0);
+    L.chunk = ci;
+    A.log[ci] = L;
+  }
+}
+
 __device__ __forceinline__ void log_chunk(const ExpandArgs& A, const HopDev& H, uint32_t ci, int var, uint64_t t0) {
   if (A.log && threadIdx.x == 0) {
     const uint64_t t1 = wall_clock64();
@@ -420,19 +459,18 @@ template <int KIND>
 __global__ __launch_bounds__(256) void k_expand_thread(GraphDev G, csaw_graph_view hv, const ReqDev* __restrict__ R,
                                                        const int32_t* __restrict__ gs2req, HopDev H, ExpandArgs A) {
   __shared__ uint32_t lds[kThreadIntCap * 256];
-  __shared__ uint32_t slot;
   const int q = CSAW_VAR_THREAD * 3 + KIND;
+  // each wave claims its own chunks; lane k handles entries k, k+64, ...
   for (;;) {
-    const ChunkCursor cc = next_chunk(A, q, &slot);
+    const ChunkCursor cc = next_chunk_wave(A, q);
     if (!cc.valid) break;
     const uint64_t t0 = wall_clock64();
     const Chunk c = H.chunks[cc.ci];
-    for (int k = threadIdx.x; k < c.len; k += 256) {
+    for (int k = lane_id(); k < c.len; k += kWave) {
       const Entry en = load_entry(G, R, gs2req, H, H.order[c.start + k]);
       expand_entry_thread<KIND>(G, hv, H, en, lds + threadIdx.x, 256);
     }
-    __syncthreads();
-    log_chunk(A, H, cc.ci, CSAW_VAR_THREAD, t0);
+    log_chunk_wave(A, H, cc.ci, CSAW_VAR_THREAD, t0);
   }
 }
 
@@ -441,24 +479,20 @@ __global__ __launch_bounds__(256) void k_expand_wave(GraphDev G, csaw_graph_view
                                                      const int32_t* __restrict__ gs2req, HopDev H, ExpandArgs A) {
   __shared__ uint64_t cdf[KIND == kInt ? 4 * kWaveIntCap : 1];
   __shared__ uint32_t bms[BITMAP ? 4 * (kWaveBitmapCap / 32) : 1];
-  __shared__ uint32_t slot;
   const int var = BITMAP ? CSAW_VAR_WAVE_BITMAP : CSAW_VAR_WAVE_HASH;
   const int q = var * 3 + KIND;
   const int wid = threadIdx.x / kWave;
+  // each wave claims whole chunks and expands them 64 entries per batch
   for (;;) {
-    const ChunkCursor cc = next_chunk(A, q, &slot);
+    const ChunkCursor cc = next_chunk_wave(A, q);
     if (!cc.valid) break;
     const uint64_t t0 = wall_clock64();
     const Chunk c = H.chunks[cc.ci];
-    // each wave takes a contiguous quarter of the chunk, 64 entries per batch
-    const int per = (c.len + 3) / 4;
-    const int k_end = min((int)c.len, (wid + 1) * per);
-    for (int k0 = wid * per; k0 < k_end; k0 += kWave)
-      wave_expand_batch<KIND>(G, hv, R, gs2req, H, H.order, (int64_t)c.start + k0, min(kWave, k_end - k0),
+    for (int k0 = 0; k0 < c.len; k0 += kWave)
+      wave_expand_batch<KIND>(G, hv, R, gs2req, H, H.order, (int64_t)c.start + k0, min(kWave, (int)c.len - k0),
                               KIND == kInt ? cdf + wid * kWaveIntCap : nullptr,
                               BITMAP ? bms + wid * (kWaveBitmapCap / 32) : nullptr);
-    __syncthreads();
-    log_chunk(A, H, cc.ci, var, t0);
+    log_chunk_wave(A, H, cc.ci, var, t0);
   }
 }
 
@@ -526,12 +560,11 @@ __global__ __launch_bounds__(256) void k_expand_block(GraphDev G, csaw_graph_vie
 // copy-all chunks (d <= F without replacement): one lane per entry
 __global__ __launch_bounds__(256) void k_expand_copy(GraphDev G, csaw_graph_view hv, const ReqDev* __restrict__ R,
                                                      const int32_t* __restrict__ gs2req, HopDev H, ExpandArgs A) {
-  __shared__ uint32_t slot;
   for (;;) {
-    const ChunkCursor cc = next_chunk(A, kQueueCopy, &slot);
+    const ChunkCursor cc = next_chunk_wave(A, kQueueCopy);
     if (!cc.valid) break;
     const Chunk c = H.chunks[cc.ci];
-    for (int k = threadIdx.x; k < c.len; k += 256) {
+    for (int k = lane_id(); k < c.len; k += kWave) {
       const Entry en = load_entry(G, R, gs2req, H, H.order[c.start + k]);
       for (int32_t i = 0; i < en.d; ++i) write_edge(G, hv, H, en, en.pos + i, i);
     }

@@ -75,7 +75,7 @@ struct TimingHook {
   virtual ~TimingHook() {}
 };
 
-__device__ int mlp_choose(const ChooseArgs& A, const Chunk& c, int kind, const HopDev& H);
+__device__ int mlp_choose(const ChooseArgs& A, const float* wl, const Chunk& c, int kind, const HopDev& H);
 
 int launch_count(const LaunchCfg& L, const GraphDev& G, const ReqDev* R, const int32_t* gs2req, const HopDev& H,
                  bool bin, int64_t max_entries);

@@ -31,7 +31,7 @@ extern "C" int csaw_plan_chunk_log(const csaw_plan* p, float* features, int32_t*
       const Chunk& c = hl.chunks[i];
       if (c.bucket == kBucketCopy) continue;
       if (k < max_n) {
-        const int kind = (c.bucket - 1) >> 5;
+        const int kind = bucket_kind(c.bucket);
         if (features) chunk_features(c.len, c.sum_d, c.max_d, c.sum_f, c.max_f, kind, c.hop, c.bucket, nch,
                                      features + k * kMlpIn);
         if (variant) variant[k] = hl.log[i].variant;
