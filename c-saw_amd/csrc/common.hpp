@@ -64,6 +64,8 @@ struct GraphDev {
   const int64_t* hub_pool;      // pool offset per hub (n_hubs + 1)
   const uint64_t* hub_cdf[2];   // [DEGREE, USER]: inclusive uint64 CDFs of hubs
   const uint64_t* int_cdf[2];   // full per-entry CDF cache (null when not used)
+  const float* cdf_idx;         // 16-ary search index over cdf (null when not used)
+  const uint64_t* int_idx[2];   // 16-ary search indexes over int_cdf
   int32_t n_hubs;
   int64_t V, E;
 };

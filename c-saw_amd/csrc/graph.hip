@@ -9,6 +9,7 @@
 #include <mutex>
 #include <vector>
 
+#include "cdf_index.hpp"
 #include "graph.hpp"
 
 namespace csaw {
@@ -253,7 +254,10 @@ int graph_prepare_bias(csaw_graph* g, int32_t bias) {
       return CSAW_ERR_INVALID_ARG;
     }
     if (!g->cdf) {
-      CSAW_HIP(hipMalloc(&g->cdf, sizeof(float) * (g->E > 0 ? g->E : 1)));
+      // padded to 16 entries: the search index reads aligned 16-entry blocks
+      const int64_t n = pad16(g->E > 0 ? g->E : 1);
+      CSAW_HIP(hipMalloc(&g->cdf, sizeof(float) * n));
+      CSAW_HIP(hipMemsetAsync(g->cdf, 0, sizeof(float) * n, g->stream));
       if (g->V > 0) {
         k_fp32_cdf<<<grid_for(g->V, 256, g->num_cu), 256, 0, g->stream>>>(g->off, g->w, g->V, g->cdf);
         CSAW_LAUNCH_CHECK();
@@ -378,7 +382,9 @@ int graph_prepare_int_cache(csaw_graph* g, int32_t bias) {
   const int kind = bias == CSAW_BIAS_DEGREE ? 0 : 1;
   if (g->int_cdf[kind]) return CSAW_OK;
   CSAW_TRY(graph_prepare_bias(g, bias));
-  CSAW_HIP(hipMalloc(&g->int_cdf[kind], sizeof(uint64_t) * (g->E > 0 ? g->E : 1)));
+  const int64_t n = pad16(g->E > 0 ? g->E : 1);
+  CSAW_HIP(hipMalloc(&g->int_cdf[kind], sizeof(uint64_t) * n));
+  CSAW_HIP(hipMemsetAsync(g->int_cdf[kind], 0, sizeof(uint64_t) * n, g->stream));
   GraphDev G = g->view();
   csaw_graph_view hv = g->hook_view();
   if (g->V > 0) {
@@ -392,6 +398,49 @@ int graph_prepare_int_cache(csaw_graph* g, int32_t bias) {
     CSAW_LAUNCH_CHECK();
   }
   CSAW_HIP(hipStreamSynchronize(g->stream));
+  return CSAW_OK;
+}
+
+// Level l of the search index: K[b] = C[min(16^l (b+1) - 1, E-1)] for b < n,
+// zero for the padding b in [n, npad).
+template <typename T>
+__global__ void k_index_level(const T* __restrict__ C, int64_t E, int sh, int64_t n, int64_t npad, T* __restrict__ K) {
+  for (int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; b < npad; b += (int64_t)gridDim.x * blockDim.x) {
+    T v = 0;
+    if (b < n) {
+      const int64_t e = ((b + 1) << sh) - 1;
+      v = C[e < E ? e : E - 1];
+    }
+    K[b] = v;
+  }
+}
+
+template <typename T>
+static int build_index(const T* C, int64_t E, T** out, hipStream_t stream, int num_cu) {
+  const int64_t tot = cdf_index_size(E);
+  CSAW_HIP(hipMalloc(out, sizeof(T) * (tot > 0 ? tot : 16)));
+  int64_t at = 0;
+  for (int l = 1; l <= cdf_index_levels(E); ++l) {
+    const int sh = 4 * l;
+    const int64_t n = (E + ((int64_t)1 << sh) - 1) >> sh, npad = cdf_index_level_size(E, l);
+    k_index_level<T><<<grid_for(npad, 256, num_cu), 256, 0, stream>>>(C, E, sh, n, npad, *out + at);
+    CSAW_LAUNCH_CHECK();
+    at += npad;
+  }
+  CSAW_HIP(hipStreamSynchronize(stream));
+  return CSAW_OK;
+}
+
+int graph_prepare_index(csaw_graph* g, int32_t bias) {
+  if (bias == CSAW_BIAS_WEIGHT) {
+    CSAW_TRY(graph_prepare_bias(g, bias));
+    if (!g->cdf_idx) CSAW_TRY(build_index<float>(g->cdf, g->E, &g->cdf_idx, g->stream, g->num_cu));
+    return CSAW_OK;
+  }
+  if (bias != CSAW_BIAS_DEGREE && bias != CSAW_BIAS_USER) return CSAW_OK;
+  const int kind = bias == CSAW_BIAS_DEGREE ? 0 : 1;
+  CSAW_TRY(graph_prepare_int_cache(g, bias));
+  if (!g->int_idx[kind]) CSAW_TRY(build_index<uint64_t>(g->int_cdf[kind], g->E, &g->int_idx[kind], g->stream, g->num_cu));
   return CSAW_OK;
 }
 
@@ -412,6 +461,10 @@ csaw::GraphDev csaw_graph::view(bool use_cache) const {
   G.hub_cdf[1] = hub_cdf[1];
   G.int_cdf[0] = use_cache ? int_cdf[0] : nullptr;
   G.int_cdf[1] = use_cache ? int_cdf[1] : nullptr;
+  // an integer cache is searched through its index: both or neither
+  G.int_idx[0] = use_cache ? int_idx[0] : nullptr;
+  G.int_idx[1] = use_cache ? int_idx[1] : nullptr;
+  G.cdf_idx = use_cache ? cdf_idx : nullptr;
   G.n_hubs = n_hubs < 0 ? 0 : n_hubs;
   G.V = V;
   G.E = E;
@@ -442,6 +495,9 @@ static void graph_free(csaw_graph* g) {
   (void)hipFree(g->hub_cdf[1]);
   (void)hipFree(g->int_cdf[0]);
   (void)hipFree(g->int_cdf[1]);
+  (void)hipFree(g->int_idx[0]);
+  (void)hipFree(g->int_idx[1]);
+  (void)hipFree(g->cdf_idx);
   if (g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
 }

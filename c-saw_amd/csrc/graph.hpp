@@ -18,6 +18,9 @@ struct csaw_graph {
   int64_t* hub_pool = nullptr;  // n_hubs+1 pool offsets
   uint64_t* hub_cdf[2] = {nullptr, nullptr};  // [DEGREE, USER]
   uint64_t* int_cdf[2] = {nullptr, nullptr};  // full per-entry CDF cache [DEGREE, USER]
+  // 16-ary search indexes over cdf and int_cdf (CDF arrays padded to 16)
+  float* cdf_idx = nullptr;
+  uint64_t* int_idx[2] = {nullptr, nullptr};
   int64_t hub_entries = 0;
   hipStream_t stream = nullptr;
   int device = 0;
@@ -32,6 +35,9 @@ namespace csaw {
 // integer hub cache (CSAW_BIAS_DEGREE / CSAW_BIAS_USER).
 int graph_prepare_bias(csaw_graph* g, int32_t bias);
 int graph_prepare_int_cache(csaw_graph* g, int32_t bias);
+// Builds the search index of the CDF a bias uses from the cache (FP32 CDF for
+// CSAW_BIAS_WEIGHT, the integer CDF cache otherwise), building that CDF first.
+int graph_prepare_index(csaw_graph* g, int32_t bias);
 int hub_cdf_build(const GraphDev& G, const csaw_graph_view& hv, int kind, const std::vector<int64_t>& pool,
                   uint64_t* out, hipStream_t stream, int num_cu);
 int rmat_generate_device(int scale, int edge_factor, double a, double b, double c, uint64_t key, hipStream_t stream,

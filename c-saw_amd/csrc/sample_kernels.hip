@@ -268,7 +268,7 @@ __global__ __launch_bounds__(256) void k_expand_plain(GraphDev G, csaw_graph_vie
   const int64_t bs = min((int64_t)kWave, max((int64_t)1, (n + nw - 1) / nw));
   for (int64_t e0 = (blockIdx.x * 4 + wid) * bs; e0 < n; e0 += nw * bs) {
     const int cnt = (int)min(bs, n - e0);
-    wave_expand_batch<KIND>(G, hv, R, gs2req, H, nullptr, e0, cnt, KIND == kInt ? cdf + wid * kWaveIntCap : nullptr,
+    wave_expand_batch<KIND, false>(G, hv, R, gs2req, H, nullptr, e0, cnt, KIND == kInt ? cdf + wid * kWaveIntCap : nullptr,
                             bms + wid * (kWaveBitmapCap / 32));
   }
 }
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(256) void k_expand_list(GraphDev G, csaw_graph_view
   const int64_t bs = min((int64_t)kWave, max((int64_t)1, (n + nw - 1) / nw));
   for (int64_t e0 = (blockIdx.x * 4 + wid) * bs; e0 < n; e0 += nw * bs) {
     const int cnt = (int)min(bs, n - e0);
-    wave_expand_batch<KIND>(G, hv, R, gs2req, H, list, e0, cnt, KIND == kInt ? cdf + wid * kWaveIntCap : nullptr,
+    wave_expand_batch<KIND, false>(G, hv, R, gs2req, H, list, e0, cnt, KIND == kInt ? cdf + wid * kWaveIntCap : nullptr,
                             bms + wid * (kWaveBitmapCap / 32));
   }
 }
@@ -489,7 +489,7 @@ __global__ __launch_bounds__(256) void k_expand_wave(GraphDev G, csaw_graph_view
     const uint64_t t0 = wall_clock64();
     const Chunk c = H.chunks[cc.ci];
     for (int k0 = 0; k0 < c.len; k0 += kWave)
-      wave_expand_batch<KIND>(G, hv, R, gs2req, H, H.order, (int64_t)c.start + k0, min(kWave, (int)c.len - k0),
+      wave_expand_batch<KIND, true>(G, hv, R, gs2req, H, H.order, (int64_t)c.start + k0, min(kWave, (int)c.len - k0),
                               KIND == kInt ? cdf + wid * kWaveIntCap : nullptr,
                               BITMAP ? bms + wid * (kWaveBitmapCap / 32) : nullptr);
     log_chunk_wave(A, H, cc.ci, var, t0);
@@ -543,12 +543,13 @@ __global__ __launch_bounds__(256) void k_expand_block(GraphDev G, csaw_graph_vie
           __syncthreads();
         }
         pk.C = cdf;
+        pk.o = 0;
         pk.T = carry;
-        pk.mode = pk.T > 0 ? 1 : 0;
+        pk.mode = pk.T > 0 ? kPickLds : kPickUniform;
       }
       __syncthreads();
       if (wid == 0) {
-        const int32_t res = wave_select(en, pk, use_bm ? bms : nullptr);
+        const int32_t res = wave_select<true>(G.E, en, pk, use_bm ? bms : nullptr);
         if (lane < en.F) write_edge(G, hv, H, en, en.pos + lane, res);
       }
       __syncthreads();

@@ -15,6 +15,7 @@
 // the per-vertex sequential CDF c and i = min{i: c_i > fl32(u24 * T)}
 // (ITS + binary_search, functions.cuh:160-176, 11-43).
 #pragma once
+#include "cdf_index.hpp"
 #include "common.hpp"
 
 namespace csaw {
@@ -59,49 +60,75 @@ __device__ __forceinline__ uint32_t int_bias(const GraphDev& G, const csaw_graph
   return bias == CSAW_BIAS_USER ? EdgeBias(u, &hv) : (uint32_t)G.deg[u];
 }
 
-// ITS state of one entry.
+// ITS state of one entry: the entry's CDF is C[o .. o+d) (o = 0 for LDS).
+enum PickMode {
+  kPickUniform = 0,
+  kPickLds = 1,       // integer CDF built in LDS, binary search
+  kPickInt = 2,       // integer CDF in global memory (hub cache), binary search
+  kPickF32 = 3,       // FP32 CDF, binary search
+  kPickIntIdx = 4,    // integer CDF cache through its 16-ary index K
+  kPickF32Idx = 5,    // FP32 CDF through its 16-ary index K
+};
+
 struct Picker {
-  int mode;            // 0 uniform, 1 integer CDF in LDS, 2 integer CDF in global, 3 FP32 CDF
+  int mode;
   int32_t d;
   uint64_t T;
   float Tf;
-  const uint64_t* C;   // LDS (mode 1) or global hub CDF (mode 2)
-  const float* Cf;
+  const void* C;
+  const void* K;
+  int64_t o;
 
-  __device__ __forceinline__ int32_t pick(const U4& o) const {
-    const uint64_t t = philox_t64(o);
-    if (mode == 0) return (int32_t)mulhi64(t, (uint64_t)d);
-    if (mode == 3) {
-      const float x = __fmul_rn(philox_u24(o), Tf);
-      return (int32_t)upper_bound_dev<float>(Cf, d, x);
+  // E: entries of the CDF array (sizes the index levels).  IDX = false
+  // compiles the index modes out (callers whose graph view has no index).
+  template <bool IDX>
+  __device__ __forceinline__ int32_t pick(int64_t E, const U4& r) const {
+    const uint64_t t = philox_t64(r);
+    if (mode == kPickUniform) return (int32_t)mulhi64(t, (uint64_t)d);
+    if (mode == kPickF32 || mode == kPickF32Idx) {
+      const float x = __fmul_rn(philox_u24(r), Tf);
+      const float* Cf = static_cast<const float*>(C);
+      if constexpr (IDX)
+        if (mode == kPickF32Idx) return cdf_index_search<float>(Cf, static_cast<const float*>(K), E, o, d, x);
+      return (int32_t)upper_bound_dev<float>(Cf + o, d, x);
     }
     const uint64_t x = mulhi64(t, T);
-    return (int32_t)upper_bound_dev<uint64_t>(C, d, x);
+    const uint64_t* Ci = static_cast<const uint64_t*>(C);
+    if constexpr (IDX)
+      if (mode == kPickIntIdx) return cdf_index_search<uint64_t>(Ci, static_cast<const uint64_t*>(K), E, o, d, x);
+    return (int32_t)upper_bound_dev<uint64_t>(Ci + o, d, x);
   }
 };
 
-// Picker for a bias that needs no per-entry build (uniform, FP32, hub cache).
-// Returns false when an integer CDF must be built by the caller.
+// Picker for a bias that needs no per-entry build (uniform, FP32, CDF cache,
+// hub cache).  Returns false when an integer CDF must be built by the caller.
 __device__ __forceinline__ bool picker_global(const GraphDev& G, const Entry& en, int kind, int cap, Picker& pk) {
   pk.d = en.d;
-  pk.mode = 0;
+  pk.mode = kPickUniform;
+  pk.o = en.o;
+  pk.K = nullptr;
   if (kind == kUniform) return true;
   if (kind == kF32) {
-    pk.Cf = G.cdf + en.o;
-    pk.Tf = pk.Cf[en.d - 1];
-    if (pk.Tf > 0.0f) pk.mode = 3;
+    pk.C = G.cdf;
+    pk.K = G.cdf_idx;
+    pk.Tf = G.cdf[en.o + en.d - 1];
+    if (pk.Tf > 0.0f) pk.mode = G.cdf_idx ? kPickF32Idx : kPickF32;
     return true;
   }
   const int ck = en.bias == CSAW_BIAS_USER ? 1 : 0;
+  const uint64_t* C;
   if (G.int_cdf[ck]) {
-    pk.C = G.int_cdf[ck] + en.o;
+    C = G.int_cdf[ck];
+    pk.K = G.int_idx[ck];
   } else {
     if (en.d <= cap) return false;
     const int32_t h = hub_find(G.hub_vid, G.n_hubs, en.v);
-    pk.C = G.hub_cdf[ck] + G.hub_pool[h];
+    C = G.hub_cdf[ck];
+    pk.o = G.hub_pool[h];
   }
-  pk.T = pk.C[en.d - 1];
-  if (pk.T > 0) pk.mode = 2;
+  pk.C = C;
+  pk.T = C[pk.o + en.d - 1];
+  if (pk.T > 0) pk.mode = pk.K ? kPickIntIdx : kPickInt;
   return true;
 }
 
@@ -129,17 +156,18 @@ __device__ __forceinline__ bool wave_taken_lane(int32_t ci, int32_t res, int jj,
   return hit;
 }
 
-__device__ int32_t wave_select(const Entry& en, const Picker& pk, uint32_t* bm) {
+template <bool IDX>
+__device__ int32_t wave_select(int64_t E, const Entry& en, const Picker& pk, uint32_t* bm) {
   const int lane = lane_id();
   const int F = en.F;
   const int32_t d = en.d;
   int32_t res = -1;
   if (!en.dedup) {
-    if (lane < F) res = pk.pick(draw(en, lane, 0));
+    if (lane < F) res = pk.template pick<IDX>(E, draw(en, lane, 0));
     return res;
   }
   int32_t cand = 0;
-  if (lane < F) cand = pk.pick(draw(en, lane, 0));
+  if (lane < F) cand = pk.template pick<IDX>(E, draw(en, lane, 0));
   // Fast path: up to the first slot whose attempt-0 draw repeats an earlier
   // slot's draw, the sequential definition accepts every attempt-0 draw, so
   // those slots are final; only the rest go through the ordered loop.
@@ -169,7 +197,7 @@ __device__ int32_t wave_select(const Entry& en, const Picker& pk, uint32_t* bm) 
         bool fr = false;
         int32_t ci = 0;
         if (a < kReselect) {
-          ci = pk.pick(draw(en, jj, a));
+          ci = pk.template pick<IDX>(E, draw(en, jj, a));
           fr = !wave_taken_lane(ci, res, jj, bm);
         }
         const uint64_t m = ballot(fr);
@@ -228,7 +256,7 @@ __device__ __forceinline__ void wave_copy_all(const GraphDev& G, const csaw_grap
 // Full wave expansion of one entry whose ITS state was prefetched (build:
 // the integer CDF must be built here).  cdf: LDS region of kWaveIntCap u64,
 // bm: LDS bitmap of kWaveBitmapCap bits (nullptr: list taken-set).
-template <int KIND>
+template <int KIND, bool IDX>
 __device__ void expand_entry_wave(const GraphDev& G, const csaw_graph_view& hv, const HopDev& H, const Entry& en,
                                   Picker pk, bool build, uint64_t* cdf, uint32_t* bm) {
   if (en.d == 0 || en.F == 0) return;
@@ -238,10 +266,11 @@ __device__ void expand_entry_wave(const GraphDev& G, const csaw_graph_view& hv, 
   }
   if (build) {
     pk.C = cdf;
+    pk.o = 0;
     pk.T = wave_build_int_cdf(G, hv, en, cdf);
-    pk.mode = pk.T > 0 ? 1 : 0;
+    pk.mode = pk.T > 0 ? kPickLds : kPickUniform;
   }
-  const int32_t res = wave_select(en, pk, (bm && en.d <= kWaveBitmapCap) ? bm : nullptr);
+  const int32_t res = wave_select<IDX>(G.E, en, pk, (bm && en.d <= kWaveBitmapCap) ? bm : nullptr);
   if (lane_id() < en.F) write_edge(G, hv, H, en, en.pos + lane_id(), res);
 }
 
@@ -271,15 +300,16 @@ __device__ __forceinline__ Picker bcast_picker(const Picker& m, int j) {
   p.d = readlane(m.d, j);
   p.T = readlane_u64(m.T, j);
   p.Tf = __int_as_float(readlane(__float_as_int(m.Tf), j));
-  p.C = (const uint64_t*)readlane_u64((uint64_t)m.C, j);
-  p.Cf = (const float*)readlane_u64((uint64_t)m.Cf, j);
+  p.C = (const void*)readlane_u64((uint64_t)m.C, j);
+  p.K = (const void*)readlane_u64((uint64_t)m.K, j);
+  p.o = readlane_i64(m.o, j);
   return p;
 }
 
 // Expands `count` (<= 64) entries with one wave: lane i loads entry i's
 // metadata and ITS state (one round of dependent loads for all of them),
 // then the wave expands the entries in order from broadcast values.
-template <int KIND>
+template <int KIND, bool IDX>
 __device__ void wave_expand_batch(const GraphDev& G, const csaw_graph_view& hv, const ReqDev* __restrict__ R,
                                   const int32_t* __restrict__ gs2req, const HopDev& H,
                                   const int32_t* __restrict__ order, int64_t first, int count, uint64_t* cdf,
@@ -292,7 +322,8 @@ __device__ void wave_expand_batch(const GraphDev& G, const csaw_graph_view& hv, 
   pm.T = 0;
   pm.Tf = 0.f;
   pm.C = nullptr;
-  pm.Cf = nullptr;
+  pm.K = nullptr;
+  pm.o = 0;
   int build = 0;
   mine.d = 0;
   if (lane < count) {
@@ -303,7 +334,7 @@ __device__ void wave_expand_batch(const GraphDev& G, const csaw_graph_view& hv, 
   for (int j = 0; j < count; ++j) {
     const Entry en = bcast_entry(mine, j);
     const Picker pk = bcast_picker(pm, j);
-    expand_entry_wave<KIND>(G, hv, H, en, pk, readlane(build, j) != 0, cdf, bm);
+    expand_entry_wave<KIND, IDX>(G, hv, H, en, pk, readlane(build, j) != 0, cdf, bm);
   }
 }
 
@@ -321,34 +352,43 @@ __device__ void expand_entry_thread(const GraphDev& G, const csaw_graph_view& hv
     for (int32_t i = 0; i < d; ++i) write_edge(G, hv, H, en, en.pos + i, i);
     return;
   }
-  // picker
-  int mode = 0;
+  // picker (PickMode; kPickLds here is a linear scan over LDS weights).
+  // Cf / Cg point at the entry's CDF, Kf / Kg at the index of the whole array.
+  int mode = kPickUniform;
   uint64_t T = 0;
   float Tf = 0.f;
   const float* Cf = nullptr;
   const uint64_t* Cg = nullptr;
+  const float* Kf = nullptr;
+  const uint64_t* Kg = nullptr;
+  const int ck = en.bias == CSAW_BIAS_USER ? 1 : 0;
   if (KIND == kF32) {
     Cf = G.cdf + en.o;
+    Kf = G.cdf_idx;
     Tf = Cf[d - 1];
-    if (Tf > 0.0f) mode = 3;
-  } else if (KIND == kInt && G.int_cdf[en.bias == CSAW_BIAS_USER ? 1 : 0]) {
-    Cg = G.int_cdf[en.bias == CSAW_BIAS_USER ? 1 : 0] + en.o;
+    if (Tf > 0.0f) mode = Kf ? kPickF32Idx : kPickF32;
+  } else if (KIND == kInt && G.int_cdf[ck]) {
+    Cg = G.int_cdf[ck] + en.o;
+    Kg = G.int_idx[ck];
     T = Cg[d - 1];
-    if (T > 0) mode = 2;
+    if (T > 0) mode = Kg ? kPickIntIdx : kPickInt;
   } else if (KIND == kInt) {
     for (int32_t i = 0; i < d; ++i) {
       const uint32_t w = int_bias(G, hv, en.bias, G.adj[en.o + i]);
       lds[i * stride] = w;
       T += w;
     }
-    if (T > 0) mode = 1;
+    if (T > 0) mode = kPickLds;
   }
   auto pick = [&](const U4& o) -> int32_t {
     const uint64_t t = philox_t64(o);
-    if (mode == 0) return (int32_t)mulhi64(t, (uint64_t)d);
-    if (mode == 3) return (int32_t)upper_bound_dev<float>(Cf, d, __fmul_rn(philox_u24(o), Tf));
+    if (mode == kPickUniform) return (int32_t)mulhi64(t, (uint64_t)d);
+    if (mode == kPickF32) return (int32_t)upper_bound_dev<float>(Cf, d, __fmul_rn(philox_u24(o), Tf));
+    if (mode == kPickF32Idx)
+      return cdf_index_search<float>(G.cdf, Kf, G.E, en.o, d, __fmul_rn(philox_u24(o), Tf));
     const uint64_t x = mulhi64(t, T);
-    if (mode == 2) return (int32_t)upper_bound_dev<uint64_t>(Cg, d, x);
+    if (mode == kPickInt) return (int32_t)upper_bound_dev<uint64_t>(Cg, d, x);
+    if (mode == kPickIntIdx) return cdf_index_search<uint64_t>(G.int_cdf[ck], Kg, G.E, en.o, d, x);
     uint64_t acc = 0;
     int32_t i = 0;
     for (; i < d - 1; ++i) {
@@ -361,7 +401,24 @@ __device__ void expand_entry_thread(const GraphDev& G, const csaw_graph_view& hv
   // each lane keeps up to four independent loads in flight.
   constexpr int G4 = 4;
   auto pick4 = [&](int j0, int nq, int32_t* out) {
-    if (mode == 2 || mode == 3) {
+    // (the index searches are instantiated only for their own kind)
+    if (KIND == kInt && mode == kPickIntIdx) {
+      if constexpr (KIND == kInt) {
+        // 64-bit keys: two pairs in lockstep (16 keys of a search take 32 registers)
+        uint64_t xi[G4];
+#pragma unroll
+        for (int q = 0; q < G4; ++q) xi[q] = q < nq ? mulhi64(philox_t64(draw(en, j0 + q, 0)), T) : 0;
+        cdf_index_search_n<uint64_t, 2>(G.int_cdf[ck], Kg, G.E, en.o, d, xi, min(nq, 2), out);
+        if (nq > 2) cdf_index_search_n<uint64_t, 2>(G.int_cdf[ck], Kg, G.E, en.o, d, xi + 2, nq - 2, out + 2);
+      }
+    } else if (KIND == kF32 && mode == kPickF32Idx) {
+      if constexpr (KIND == kF32) {
+        float xf[G4];
+#pragma unroll
+        for (int q = 0; q < G4; ++q) xf[q] = q < nq ? __fmul_rn(philox_u24(draw(en, j0 + q, 0)), Tf) : 0.f;
+        cdf_index_search_n<float, G4>(G.cdf, Kf, G.E, en.o, d, xf, nq, out);
+      }
+    } else if (mode == kPickInt || mode == kPickF32) {
       int32_t lo[G4], hi[G4];
       uint64_t xi[G4];
       float xf[G4];
@@ -373,7 +430,7 @@ __device__ void expand_entry_thread(const GraphDev& G, const csaw_graph_view& hv
         xf[q] = 0.f;
         if (q < nq) {
           const U4 o = draw(en, j0 + q, 0);
-          if (mode == 2) xi[q] = mulhi64(philox_t64(o), T);
+          if (mode == kPickInt) xi[q] = mulhi64(philox_t64(o), T);
           else xf[q] = __fmul_rn(philox_u24(o), Tf);
         }
       }
@@ -387,7 +444,7 @@ __device__ void expand_entry_thread(const GraphDev& G, const csaw_graph_view& hv
           mid[q] = (lo[q] + hi[q]) >> 1;
           if (lo[q] < hi[q]) {
             any = true;
-            if (mode == 2) vi[q] = Cg[mid[q]];
+            if (mode == kPickInt) vi[q] = Cg[mid[q]];
             else vf[q] = Cf[mid[q]];
           }
         }
@@ -395,7 +452,7 @@ __device__ void expand_entry_thread(const GraphDev& G, const csaw_graph_view& hv
 #pragma unroll
         for (int q = 0; q < G4; ++q) {
           if (lo[q] < hi[q]) {
-            const bool gt = mode == 2 ? vi[q] > xi[q] : vf[q] > xf[q];
+            const bool gt = mode == kPickInt ? vi[q] > xi[q] : vf[q] > xf[q];
             if (gt) hi[q] = mid[q]; else lo[q] = mid[q] + 1;
           }
         }

@@ -22,12 +22,21 @@ __device__ __forceinline__ int32_t walk_pick(const GraphDev& G, const csaw_graph
                                              int64_t o, int32_t d, const U4& r) {
   const uint64_t t = philox_t64(r);
   if (KIND == kF32) {
-    const float* C = G.cdf + o;
-    const float T = C[d - 1];
-    if (T > 0.0f) return (int32_t)upper_bound_dev<float>(C, d, __fmul_rn(philox_u24(r), T));
+    const float T = G.cdf[o + d - 1];
+    if (T > 0.0f) {
+      const float x = __fmul_rn(philox_u24(r), T);
+      if (G.cdf_idx) return cdf_index_search<float>(G.cdf, G.cdf_idx, G.E, o, d, x);
+      return (int32_t)upper_bound_dev<float>(G.cdf + o, d, x);
+    }
     return (int32_t)mulhi64(t, (uint64_t)d);
   }
   if (KIND == kInt) {
+    const int ck = bias == CSAW_BIAS_USER ? 1 : 0;
+    if (G.int_cdf[ck] && G.int_idx[ck]) {
+      const uint64_t T = G.int_cdf[ck][o + d - 1];
+      if (T > 0) return cdf_index_search<uint64_t>(G.int_cdf[ck], G.int_idx[ck], G.E, o, d, mulhi64(t, T));
+      return (int32_t)mulhi64(t, (uint64_t)d);
+    }
     if (d > kHubMin) {
       const int32_t h = hub_find(G.hub_vid, G.n_hubs, v);
       const uint64_t* C = G.hub_cdf[bias == CSAW_BIAS_USER ? 1 : 0] + G.hub_pool[h];
@@ -81,8 +90,10 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk(GraphDev G, csaw_graph_vi
             // walk entry 4, plus the CDF total and ceil(log2 d) probes (FP32 /
             // cached integer) or two passes over adj + degree (integer, built)
             const int l = ceil_log2_u32((uint32_t)d);
+            const bool cached = KIND == kInt && G.int_cdf[bias == CSAW_BIAS_USER ? 1 : 0];
             bytes += 20 + (KIND == kF32 ? 4 + 4 * l
-                                        : (KIND == kInt ? (d > kHubMin ? 8 + 8 * l : 16 * (unsigned long long)d) : 0));
+                                        : (KIND == kInt ? (cached || d > kHubMin ? 8 + 8 * l : 16 * (unsigned long long)d)
+                                                        : 0));
           }
         }
         tile[k][threadIdx.x] = v;
@@ -150,7 +161,8 @@ extern "C" int csaw_walk_plan_create(csaw_graph* g, int32_t bias, uint64_t seed,
   *out = nullptr;
   for (int64_t i = 0; i < n_walkers; ++i)
     if (seeds[i] < 0 || seeds[i] >= g->V) return CSAW_ERR_INVALID_ARG;
-  CSAW_TRY(graph_prepare_bias(g, bias));
+  // walks search the CDF cache (FP32 CDF or integer cache) through its index
+  CSAW_TRY(graph_prepare_index(g, bias));
   csaw_walk_plan* p = new csaw_walk_plan();
   p->g = g;
   p->bias = bias;
@@ -187,7 +199,7 @@ extern "C" int csaw_walk_plan_run(csaw_walk_plan* p, void* stream, int sync) {
   CSAW_HIP(hipEventCreate(&e1));
   CSAW_HIP(hipMemsetAsync(p->d_steps, 0, 2 * sizeof(unsigned long long), st));
   CSAW_HIP(hipEventRecord(e0, st));
-  CSAW_TRY(launch_walk(st, p->g->num_cu, p->per_cu, p->g->view(), p->g->hook_view(), p->bias, p->key, p->request,
+  CSAW_TRY(launch_walk(st, p->g->num_cu, p->per_cu, p->g->view(true), p->g->hook_view(), p->bias, p->key, p->request,
                        p->d_seeds, p->n, p->len, p->d_W, p->d_steps));
   CSAW_HIP(hipEventRecord(e1, st));
   if (sync) {
