@@ -66,6 +66,8 @@ struct GraphDev {
   const uint64_t* int_cdf[2];   // full per-entry CDF cache (null when not used)
   const float* cdf_idx;         // 16-ary search index over cdf (null when not used)
   const uint64_t* int_idx[2];   // 16-ary search indexes over int_cdf
+  const uint32_t* int_cdf32[2]; // the cache as uint32 when all totals fit (int_cdf null then)
+  const uint32_t* int_idx32[2];
   int32_t n_hubs;
   int64_t V, E;
 };

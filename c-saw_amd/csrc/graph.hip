@@ -377,10 +377,26 @@ __global__ void k_int_cdf_hubs(GraphDev G, const uint64_t* __restrict__ pool, ui
   }
 }
 
+__global__ void k_max_u64(const uint64_t* __restrict__ C, int64_t n, unsigned long long* __restrict__ out) {
+  unsigned long long m = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    m = C[i] > m ? C[i] : m;
+  for (int off = 32; off >= 1; off >>= 1) {
+    const unsigned long long o = __shfl_xor(m, off, 64);
+    m = o > m ? o : m;
+  }
+  if (lane_id() == 0 && m) atomicMax(out, m);
+}
+
+__global__ void k_narrow_u32(const uint64_t* __restrict__ C, int64_t n, uint32_t* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (uint32_t)C[i];
+}
+
 int graph_prepare_int_cache(csaw_graph* g, int32_t bias) {
   if (bias != CSAW_BIAS_DEGREE && bias != CSAW_BIAS_USER) return CSAW_OK;
   const int kind = bias == CSAW_BIAS_DEGREE ? 0 : 1;
-  if (g->int_cdf[kind]) return CSAW_OK;
+  if (g->int_cdf[kind] || g->int_cdf32[kind]) return CSAW_OK;
   CSAW_TRY(graph_prepare_bias(g, bias));
   const int64_t n = pad16(g->E > 0 ? g->E : 1);
   CSAW_HIP(hipMalloc(&g->int_cdf[kind], sizeof(uint64_t) * n));
@@ -397,7 +413,27 @@ int graph_prepare_int_cache(csaw_graph* g, int32_t bias) {
     k_int_cdf_hubs<<<std::min(g->n_hubs, g->num_cu * 8), 256, 0, g->stream>>>(G, g->hub_cdf[kind], g->int_cdf[kind]);
     CSAW_LAUNCH_CHECK();
   }
+  // Every per-vertex total below 2^32 (the largest entry is the largest
+  // total): keep the cache as uint32, half the bytes per search.
+  unsigned long long* d_max = nullptr;
+  CSAW_HIP(hipMalloc(&d_max, sizeof(unsigned long long)));
+  CSAW_HIP(hipMemsetAsync(d_max, 0, sizeof(unsigned long long), g->stream));
+  if (g->E > 0) {
+    k_max_u64<<<grid_for(g->E, 256, g->num_cu), 256, 0, g->stream>>>(g->int_cdf[kind], g->E, d_max);
+    CSAW_LAUNCH_CHECK();
+  }
+  unsigned long long mx = 0;
+  CSAW_HIP(hipMemcpyAsync(&mx, d_max, sizeof(mx), hipMemcpyDeviceToHost, g->stream));
   CSAW_HIP(hipStreamSynchronize(g->stream));
+  CSAW_HIP(hipFree(d_max));
+  if (mx < (1ull << 32)) {
+    CSAW_HIP(hipMalloc(&g->int_cdf32[kind], sizeof(uint32_t) * n));
+    k_narrow_u32<<<grid_for(n, 256, g->num_cu), 256, 0, g->stream>>>(g->int_cdf[kind], n, g->int_cdf32[kind]);
+    CSAW_LAUNCH_CHECK();
+    CSAW_HIP(hipStreamSynchronize(g->stream));
+    CSAW_HIP(hipFree(g->int_cdf[kind]));
+    g->int_cdf[kind] = nullptr;
+  }
   return CSAW_OK;
 }
 
@@ -440,7 +476,12 @@ int graph_prepare_index(csaw_graph* g, int32_t bias) {
   if (bias != CSAW_BIAS_DEGREE && bias != CSAW_BIAS_USER) return CSAW_OK;
   const int kind = bias == CSAW_BIAS_DEGREE ? 0 : 1;
   CSAW_TRY(graph_prepare_int_cache(g, bias));
-  if (!g->int_idx[kind]) CSAW_TRY(build_index<uint64_t>(g->int_cdf[kind], g->E, &g->int_idx[kind], g->stream, g->num_cu));
+  if (g->int_cdf32[kind]) {
+    if (!g->int_idx32[kind])
+      CSAW_TRY(build_index<uint32_t>(g->int_cdf32[kind], g->E, &g->int_idx32[kind], g->stream, g->num_cu));
+  } else if (!g->int_idx[kind]) {
+    CSAW_TRY(build_index<uint64_t>(g->int_cdf[kind], g->E, &g->int_idx[kind], g->stream, g->num_cu));
+  }
   return CSAW_OK;
 }
 
@@ -464,6 +505,10 @@ csaw::GraphDev csaw_graph::view(bool use_cache) const {
   // an integer cache is searched through its index: both or neither
   G.int_idx[0] = use_cache ? int_idx[0] : nullptr;
   G.int_idx[1] = use_cache ? int_idx[1] : nullptr;
+  for (int k = 0; k < 2; ++k) {
+    G.int_cdf32[k] = use_cache ? int_cdf32[k] : nullptr;
+    G.int_idx32[k] = use_cache ? int_idx32[k] : nullptr;
+  }
   G.cdf_idx = use_cache ? cdf_idx : nullptr;
   G.n_hubs = n_hubs < 0 ? 0 : n_hubs;
   G.V = V;
@@ -497,6 +542,10 @@ static void graph_free(csaw_graph* g) {
   (void)hipFree(g->int_cdf[1]);
   (void)hipFree(g->int_idx[0]);
   (void)hipFree(g->int_idx[1]);
+  for (int k = 0; k < 2; ++k) {
+    (void)hipFree(g->int_cdf32[k]);
+    (void)hipFree(g->int_idx32[k]);
+  }
   (void)hipFree(g->cdf_idx);
   if (g->stream) (void)hipStreamDestroy(g->stream);
   delete g;

@@ -21,6 +21,9 @@ struct csaw_graph {
   // 16-ary search indexes over cdf and int_cdf (CDF arrays padded to 16)
   float* cdf_idx = nullptr;
   uint64_t* int_idx[2] = {nullptr, nullptr};
+  // the integer cache as uint32 when every total fits (then int_cdf is null)
+  uint32_t* int_cdf32[2] = {nullptr, nullptr};
+  uint32_t* int_idx32[2] = {nullptr, nullptr};
   int64_t hub_entries = 0;
   hipStream_t stream = nullptr;
   int device = 0;

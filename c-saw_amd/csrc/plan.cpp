@@ -270,6 +270,7 @@ static int run_unit(csaw_plan* p, Unit& u, hipStream_t st, TimingHook* th) {
       E.qlen = p->counters + kCtrQlen;
       E.qhead = p->counters + kCtrQhead;
       E.log = p->opts.log_chunks ? p->log : nullptr;
+      E.int_cache = p->opts.cdf_cache != 0;
       CSAW_TRY(launch_expand_chunks(L, G, hv, u.d_req, u.d_gs2req, H, E, u.kinds, th));
       if (p->opts.log_chunks) {
         // keep this hop's chunks and log for the host
@@ -280,8 +281,13 @@ static int run_unit(csaw_plan* p, Unit& u, hipStream_t st, TimingHook* th) {
       int kind = 0;
       for (int k = 0; k < 3; ++k)
         if (u.kinds[k]) kind = k;
+      // sub-wave group size from the largest fanout of the hop
+      int fmax = 0;
+      for (int ri : u.reqs)
+        if (p->reqs[ri].depth > h) fmax = std::max(fmax, p->reqs[ri].fanouts[h]);
+      const int gs = fmax <= 8 ? 8 : (fmax <= 16 ? 16 : (fmax <= 32 ? 32 : 0));
       if (th) th->begin(kKernPlain, st);
-      CSAW_TRY(launch_expand_plain(L, kind, G, hv, u.d_req, u.d_gs2req, H, u.nmax[h]));
+      CSAW_TRY(launch_expand_plain(L, kind, G, hv, u.d_req, u.d_gs2req, H, u.nmax[h], gs));
       if (th) th->end(kKernPlain, st);
     }
     if (bin && p->opts.log_chunks) {

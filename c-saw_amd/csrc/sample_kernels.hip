@@ -64,7 +64,8 @@ __global__ __launch_bounds__(256) void k_count(GraphDev G, const ReqDev* __restr
         k[i] = (uint32_t)entry_k(d, F, r.dedup);
         if (!BIN && H.bytes) {
           const int kind = bias_kind(r.bias);
-          const bool built = kind == kInt && !G.int_cdf[r.bias == CSAW_BIAS_USER ? 1 : 0] && d <= kWaveIntCap;
+          const int ck = r.bias == CSAW_BIAS_USER ? 1 : 0;
+          const bool built = kind == kInt && !G.int_cdf[ck] && !G.int_cdf32[ck] && d <= kWaveIntCap;
           tbytes += k[i] == 0 ? 20ull
                               : expansion_bytes(kind, built, 1, d, k[i], ceil_log2_u32((uint32_t)d),
                                                 r.dedup && d <= F);
@@ -255,21 +256,39 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, int64_t n) {
 }
 
 // ----------------------------------------------------------- expand plain
+// gs: sub-wave group size for the hop's fanout (8, 16 or 32; 0 = whole waves),
+// used for the kinds that need no per-entry CDF build (uniform, FP32).
 template <int KIND>
 __global__ __launch_bounds__(256) void k_expand_plain(GraphDev G, csaw_graph_view hv, const ReqDev* __restrict__ R,
-                                                      const int32_t* __restrict__ gs2req, HopDev H) {
+                                                      const int32_t* __restrict__ gs2req, HopDev H, int gs) {
   __shared__ uint64_t cdf[KIND == kInt ? 4 * kWaveIntCap : 1];
   __shared__ uint32_t bms[4 * (kWaveBitmapCap / 32)];
   const int64_t n = *H.n_ptr;
   const int wid = threadIdx.x / kWave;
   const int64_t nw = (int64_t)gridDim.x * 4;
-  // batch size adapts to the frontier: one entry per wave when it is small
-  // (all waves busy), up to 64 entries per metadata round when it is large
-  const int64_t bs = min((int64_t)kWave, max((int64_t)1, (n + nw - 1) / nw));
+  // batch size adapts to the frontier: one entry (one group round) per wave
+  // when it is small (all waves busy), up to 64 entries per metadata round
+  // when it is large
+  const int64_t per = KIND != kInt && gs > 0 ? kWave / gs : 1;
+  const int64_t bs = min((int64_t)kWave, max(per, (n + nw - 1) / nw));
   for (int64_t e0 = (blockIdx.x * 4 + wid) * bs; e0 < n; e0 += nw * bs) {
     const int cnt = (int)min(bs, n - e0);
+    if constexpr (KIND != kInt) {
+      if (gs == 8) {
+        group_expand_batch<KIND, 8>(G, hv, R, gs2req, H, nullptr, e0, cnt);
+        continue;
+      }
+      if (gs == 16) {
+        group_expand_batch<KIND, 16>(G, hv, R, gs2req, H, nullptr, e0, cnt);
+        continue;
+      }
+      if (gs == 32) {
+        group_expand_batch<KIND, 32>(G, hv, R, gs2req, H, nullptr, e0, cnt);
+        continue;
+      }
+    }
     wave_expand_batch<KIND, false>(G, hv, R, gs2req, H, nullptr, e0, cnt, KIND == kInt ? cdf + wid * kWaveIntCap : nullptr,
-                            bms + wid * (kWaveBitmapCap / 32));
+                                   bms + wid * (kWaveBitmapCap / 32));
   }
 }
 
@@ -482,16 +501,27 @@ __global__ __launch_bounds__(256) void k_expand_wave(GraphDev G, csaw_graph_view
   const int var = BITMAP ? CSAW_VAR_WAVE_BITMAP : CSAW_VAR_WAVE_HASH;
   const int q = var * 3 + KIND;
   const int wid = threadIdx.x / kWave;
-  // each wave claims whole chunks and expands them 64 entries per batch
+  // each wave claims whole chunks and expands them 64 entries per batch;
+  // chunks whose fanouts fit a sub-wave group (and need no CDF build) run
+  // 64 / GS entries at a time
+  const bool groups_ok = KIND != kInt || A.int_cache;
   for (;;) {
     const ChunkCursor cc = next_chunk_wave(A, q);
     if (!cc.valid) break;
     const uint64_t t0 = wall_clock64();
     const Chunk c = H.chunks[cc.ci];
-    for (int k0 = 0; k0 < c.len; k0 += kWave)
-      wave_expand_batch<KIND, true>(G, hv, R, gs2req, H, H.order, (int64_t)c.start + k0, min(kWave, (int)c.len - k0),
-                              KIND == kInt ? cdf + wid * kWaveIntCap : nullptr,
-                              BITMAP ? bms + wid * (kWaveBitmapCap / 32) : nullptr);
+    const int gs = c.max_f <= 8 ? 8 : (c.max_f <= 16 ? 16 : (c.max_f <= 32 ? 32 : kWave));
+    for (int k0 = 0; k0 < c.len; k0 += kWave) {
+      const int64_t first = (int64_t)c.start + k0;
+      const int cnt = min(kWave, (int)c.len - k0);
+      if (groups_ok && gs == 8) group_expand_batch<KIND, 8>(G, hv, R, gs2req, H, H.order, first, cnt);
+      else if (groups_ok && gs == 16) group_expand_batch<KIND, 16>(G, hv, R, gs2req, H, H.order, first, cnt);
+      else if (groups_ok && gs == 32) group_expand_batch<KIND, 32>(G, hv, R, gs2req, H, H.order, first, cnt);
+      else
+        wave_expand_batch<KIND, true>(G, hv, R, gs2req, H, H.order, first, cnt,
+                                      KIND == kInt ? cdf + wid * kWaveIntCap : nullptr,
+                                      BITMAP ? bms + wid * (kWaveBitmapCap / 32) : nullptr);
+    }
     log_chunk_wave(A, H, cc.ci, var, t0);
   }
 }
@@ -611,13 +641,13 @@ int launch_fill(const LaunchCfg& L, uint32_t* p, uint32_t v, int64_t n) {
 }
 
 int launch_expand_plain(const LaunchCfg& L, int kind, const GraphDev& G, const csaw_graph_view& hv, const ReqDev* R,
-                        const int32_t* gs2req, const HopDev& H, int64_t max_entries) {
+                        const int32_t* gs2req, const HopDev& H, int64_t max_entries, int gs) {
   int64_t nwg = (max_entries + 3) / 4;
   int grid = (int)std::max<int64_t>(1, std::min<int64_t>(nwg, grid_of(L, L.expand_per_cu)));
   if (L.grid_override > 0) grid = L.grid_override;
-  if (kind == kUniform) k_expand_plain<kUniform><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H);
-  else if (kind == kInt) k_expand_plain<kInt><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H);
-  else k_expand_plain<kF32><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H);
+  if (kind == kUniform) k_expand_plain<kUniform><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, gs);
+  else if (kind == kInt) k_expand_plain<kInt><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, 0);
+  else k_expand_plain<kF32><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, gs);
   CSAW_LAUNCH_CHECK();
   return CSAW_OK;
 }

@@ -53,6 +53,7 @@ struct ExpandArgs {
   const uint32_t* qlen;
   uint32_t* qhead;
   ChunkLog* log;
+  int32_t int_cache;  // integer biases search the CDF cache (no per-entry build)
 };
 
 struct ChooseArgs {
@@ -84,7 +85,7 @@ int launch_fill(const LaunchCfg& L, uint32_t* p, uint32_t v, int64_t n);
 int launch_digest(hipStream_t st, int num_cu, const int32_t* src, const int32_t* dst, const int32_t* smp, int64_t lo,
                   int64_t hi, int64_t pos0, int64_t gbase, unsigned long long* out);
 int launch_expand_plain(const LaunchCfg& L, int kind, const GraphDev& G, const csaw_graph_view& hv, const ReqDev* R,
-                        const int32_t* gs2req, const HopDev& H, int64_t max_entries);
+                        const int32_t* gs2req, const HopDev& H, int64_t max_entries, int gs);
 int launch_choose(const LaunchCfg& L, const ChooseArgs& A, const HopDev& H, int64_t max_chunks);
 int launch_expand_list(const LaunchCfg& L, int kind, const GraphDev& G, const csaw_graph_view& hv, const ReqDev* R,
                        const int32_t* gs2req, const HopDev& H, const int32_t* list, int64_t n);

@@ -41,8 +41,9 @@ __device__ __forceinline__ Entry load_entry(const GraphDev& G, const ReqDev* __r
   en.F = r.fanout[H.hop];
   en.dedup = r.dedup;
   en.bias = r.bias;
+  // offset and degree from one pair of offsets (one line, not two)
   en.o = G.off[en.v];
-  en.d = G.deg[en.v];
+  en.d = (int32_t)(G.off[en.v + 1] - en.o);
   en.pos = (int64_t)H.grp_excl[e >> kGroupShift] + H.blk_excl[e >> kBlockShift] + H.local_excl[e];
   en.c1 = (uint32_t)(e - H.fo[en.g]);
   en.c2 = (uint32_t)(en.g - r.gbase);
@@ -68,6 +69,7 @@ enum PickMode {
   kPickF32 = 3,       // FP32 CDF, binary search
   kPickIntIdx = 4,    // integer CDF cache through its 16-ary index K
   kPickF32Idx = 5,    // FP32 CDF through its 16-ary index K
+  kPickInt32Idx = 6,  // uint32 integer CDF cache through its 16-ary index K (T < 2^32)
 };
 
 struct Picker {
@@ -93,10 +95,14 @@ struct Picker {
       return (int32_t)upper_bound_dev<float>(Cf + o, d, x);
     }
     const uint64_t x = mulhi64(t, T);
-    const uint64_t* Ci = static_cast<const uint64_t*>(C);
-    if constexpr (IDX)
-      if (mode == kPickIntIdx) return cdf_index_search<uint64_t>(Ci, static_cast<const uint64_t*>(K), E, o, d, x);
-    return (int32_t)upper_bound_dev<uint64_t>(Ci + o, d, x);
+    if constexpr (IDX) {
+      if (mode == kPickInt32Idx)
+        return cdf_index_search<uint32_t>(static_cast<const uint32_t*>(C), static_cast<const uint32_t*>(K), E, o, d,
+                                          (uint32_t)x);
+      if (mode == kPickIntIdx)
+        return cdf_index_search<uint64_t>(static_cast<const uint64_t*>(C), static_cast<const uint64_t*>(K), E, o, d, x);
+    }
+    return (int32_t)upper_bound_dev<uint64_t>(static_cast<const uint64_t*>(C) + o, d, x);
   }
 };
 
@@ -116,6 +122,13 @@ __device__ __forceinline__ bool picker_global(const GraphDev& G, const Entry& en
     return true;
   }
   const int ck = en.bias == CSAW_BIAS_USER ? 1 : 0;
+  if (G.int_cdf32[ck]) {
+    pk.C = G.int_cdf32[ck];
+    pk.K = G.int_idx32[ck];
+    pk.T = G.int_cdf32[ck][en.o + en.d - 1];
+    if (pk.T > 0) pk.mode = kPickInt32Idx;
+    return true;
+  }
   const uint64_t* C;
   if (G.int_cdf[ck]) {
     C = G.int_cdf[ck];
@@ -339,6 +352,234 @@ __device__ void wave_expand_batch(const GraphDev& G, const csaw_graph_view& hv, 
 }
 
 // ---------------------------------------------------------------------------
+// Sub-wave groups, for entries with F <= GS < 64: the wave splits into 64/GS
+// groups of GS lanes; each round, group g expands entry e0 + g of the batch
+// with lane j of the group holding slot j.  The selection is wave_select's
+// list taken-set definition with ballots masked to the group and shuffles
+// inside the group in place of readlane, so the edges are the same, with
+// 64/GS entries in flight per wave instead of one.  Every shuffle reads lanes
+// of the shuffling group only, and all lanes of a group follow the same path.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int32_t shfl_abs(int32_t v, int src) { return __builtin_amdgcn_ds_bpermute(src << 2, v); }
+__device__ __forceinline__ uint32_t shfl_abs_u(uint32_t v, int src) { return (uint32_t)shfl_abs((int32_t)v, src); }
+__device__ __forceinline__ uint64_t shfl_abs_u64(uint64_t v, int src) {
+  return (uint64_t)shfl_abs_u((uint32_t)v, src) | ((uint64_t)shfl_abs_u((uint32_t)(v >> 32), src) << 32);
+}
+
+// One frontier entry as the group path keeps it (16 dwords, so a batch of 64
+// costs 16 registers per lane, and one shuffle per dword to fetch).
+struct GEntry {
+  int32_t v, d;
+  int64_t o;
+  int32_t F, g;
+  int64_t pos;
+  uint32_t c1, c2, c3, k0, k1;
+  uint32_t flags;  // bit 0 dedup, bit 1 user-bias cache, bits 4-7 PickMode
+  uint64_t T;      // integer CDF total, or the FP32 total's bits
+};
+
+// Entry + its ITS state.  For KIND == kInt the CDF cache of the entry's bias
+// must exist (without it the entry is left on the uniform picker).
+template <int KIND>
+__device__ __forceinline__ GEntry make_gentry(const GraphDev& G, const Entry& en) {
+  GEntry ge;
+  ge.v = en.v;
+  ge.d = en.d;
+  ge.o = en.o;
+  ge.F = en.F;
+  ge.g = en.g;
+  ge.pos = en.pos;
+  ge.c1 = en.c1;
+  ge.c2 = en.c2;
+  ge.c3 = en.c3;
+  ge.k0 = en.k0;
+  ge.k1 = en.k1;
+  const uint32_t ck = en.bias == CSAW_BIAS_USER ? 1u : 0u;
+  uint32_t mode = kPickUniform;
+  ge.T = 0;
+  if (en.d > 0 && en.F > 0 && !(en.dedup && en.d <= en.F)) {
+    if (KIND == kF32) {
+      const float Tf = G.cdf[en.o + en.d - 1];
+      ge.T = __float_as_uint(Tf);
+      if (Tf > 0.0f) mode = G.cdf_idx ? kPickF32Idx : kPickF32;
+    } else if (KIND == kInt) {
+      if (G.int_cdf32[ck]) {
+        ge.T = G.int_cdf32[ck][en.o + en.d - 1];
+        if (ge.T > 0) mode = kPickInt32Idx;
+      } else if (G.int_cdf[ck]) {
+        ge.T = G.int_cdf[ck][en.o + en.d - 1];
+        if (ge.T > 0) mode = G.int_idx[ck] ? kPickIntIdx : kPickInt;
+      }
+    }
+  }
+  ge.flags = (en.dedup ? 1u : 0u) | (ck << 1) | (mode << 4);
+  return ge;
+}
+
+__device__ __forceinline__ U4 gdraw(const GEntry& en, int j, int a) {
+  return philox4x32_10(((uint32_t)j << 16) | (uint32_t)a, en.c1, en.c2, en.c3, en.k0, en.k1);
+}
+
+template <int KIND>
+__device__ __forceinline__ int32_t gpick(const GraphDev& G, const GEntry& en, const U4& r) {
+  const uint32_t mode = (en.flags >> 4) & 15u;
+  const uint64_t t = philox_t64(r);
+  if (KIND == kUniform || mode == kPickUniform) return (int32_t)mulhi64(t, (uint64_t)en.d);
+  if constexpr (KIND == kF32) {
+    const float x = __fmul_rn(philox_u24(r), __uint_as_float((uint32_t)en.T));
+    if (mode == kPickF32Idx) return cdf_index_search<float>(G.cdf, G.cdf_idx, G.E, en.o, en.d, x);
+    return (int32_t)upper_bound_dev<float>(G.cdf + en.o, en.d, x);
+  }
+  const bool user = (en.flags >> 1) & 1u;
+  const uint64_t x = mulhi64(t, en.T);
+  if (mode == kPickInt32Idx)
+    return cdf_index_search<uint32_t>(user ? G.int_cdf32[1] : G.int_cdf32[0], user ? G.int_idx32[1] : G.int_idx32[0],
+                                      G.E, en.o, en.d, (uint32_t)x);
+  const uint64_t* C = user ? G.int_cdf[1] : G.int_cdf[0];
+  if (mode == kPickIntIdx)
+    return cdf_index_search<uint64_t>(C, user ? G.int_idx[1] : G.int_idx[0], G.E, en.o, en.d, x);
+  return (int32_t)upper_bound_dev<uint64_t>(C + en.o, en.d, x);
+}
+
+__device__ __forceinline__ GEntry shfl_gentry(const GEntry& m, int src) {
+  GEntry e;
+  e.v = shfl_abs(m.v, src);
+  e.d = shfl_abs(m.d, src);
+  e.o = (int64_t)shfl_abs_u64((uint64_t)m.o, src);
+  e.F = shfl_abs(m.F, src);
+  e.g = shfl_abs(m.g, src);
+  e.pos = (int64_t)shfl_abs_u64((uint64_t)m.pos, src);
+  e.c1 = shfl_abs_u(m.c1, src);
+  e.c2 = shfl_abs_u(m.c2, src);
+  e.c3 = shfl_abs_u(m.c3, src);
+  e.k0 = shfl_abs_u(m.k0, src);
+  e.k1 = shfl_abs_u(m.k1, src);
+  e.flags = shfl_abs_u(m.flags, src);
+  e.T = shfl_abs_u64(m.T, src);
+  return e;
+}
+
+__device__ __forceinline__ void gwrite_edge(const GraphDev& G, const csaw_graph_view& hv, const HopDev& H,
+                                            const GEntry& en, int64_t p, int32_t idx) {
+  H.src[p] = en.v;
+  H.dst[p] = Update(&hv, G.adj[en.o + idx], en.v);
+  H.smp[p] = en.g;
+}
+
+// ci equal to one of the group's resolved slots 0 .. jj-1 (res of lane gbase+m)
+__device__ __forceinline__ bool group_taken(int32_t ci, int32_t res, int jj, int gbase) {
+  bool hit = false;
+  for (int m = 0; m < jj; ++m) hit |= (ci == shfl_abs(res, gbase + m));
+  return hit;
+}
+
+template <int KIND, int GS>
+__device__ int32_t group_select(const GraphDev& G, const GEntry& en, int gl, int gbase, uint64_t gmask) {
+  const int F = en.F;
+  const int32_t d = en.d;
+  const bool act = gl < F;
+  int32_t cand = -1;
+  if (act) cand = gpick<KIND>(G, en, gdraw(en, gl, 0));
+  if (!(en.flags & 1u)) return cand;
+  bool dup = false;
+  for (int i = 0; i + 1 < F; ++i) {
+    const int32_t ci = shfl_abs(cand, gbase + i);
+    dup |= (gl > i) & act & (cand == ci);
+  }
+  const uint64_t dm = ballot(dup) & gmask;
+  if (dm == 0ull) return cand;
+  const int j0 = __builtin_ctzll(dm) - gbase;
+  int32_t res = gl < j0 ? cand : -1;
+  for (int jj = j0; jj < F; ++jj) {
+    int32_t c = shfl_abs(cand, gbase + jj);
+    if ((ballot(res == c) & gmask) != 0ull) {
+      int32_t chosen = -1, last = c;
+      for (int a0 = 1; a0 < kReselect; a0 += GS) {
+        const int a = a0 + gl;
+        int32_t ci = 0;
+        if (a < kReselect) ci = gpick<KIND>(G, en, gdraw(en, jj, a));
+        const bool tk = group_taken(ci, res, jj, gbase);
+        const uint64_t m = ballot((a < kReselect) && !tk) & gmask;
+        if (m) {
+          chosen = shfl_abs(ci, __builtin_ctzll(m));
+          break;
+        }
+        const int ll = kReselect - 1 - a0;
+        if (ll < GS) last = shfl_abs(ci, gbase + ll);
+      }
+      if (chosen < 0) {
+        for (int q = 0; q < d; q += GS) {
+          const int32_t i = (int32_t)(((int64_t)last + 1 + q + gl) % d);
+          const bool tk = group_taken(i, res, jj, gbase);
+          const uint64_t m = ballot((q + gl < d) && !tk) & gmask;
+          if (m) {
+            chosen = shfl_abs(i, __builtin_ctzll(m));
+            break;
+          }
+        }
+      }
+      c = chosen;
+    }
+    if (gl == jj) res = c;
+  }
+  return res;
+}
+
+// Expands `count` (<= 64) entries, all with F <= GS, in groups of GS lanes.
+// Needs no per-entry CDF build: uniform, FP32, or integer biases with the
+// CDF cache.
+template <int KIND, int GS>
+__device__ void group_expand_batch(const GraphDev& G, const csaw_graph_view& hv, const ReqDev* __restrict__ R,
+                                   const int32_t* __restrict__ gs2req, const HopDev& H,
+                                   const int32_t* __restrict__ order, int64_t first, int count) {
+  constexpr int NG = kWave / GS;
+  const int lane = lane_id();
+  const int gl = lane & (GS - 1), gbase = lane & ~(GS - 1);
+  const uint64_t gmask = (GS >= kWave ? ~0ull : ((1ull << GS) - 1ull)) << gbase;
+  GEntry mine;
+  mine.d = 0;
+  mine.F = 0;
+  if (lane < count)
+    mine = make_gentry<KIND>(G, load_entry(G, R, gs2req, H, order ? (int64_t)order[first + lane] : first + lane));
+  // Software pipeline: a round's neighbour load is consumed (written out) only
+  // after the next round's selection, so its latency overlaps that round's
+  // searches instead of adding to them.
+  bool have = false;
+  int32_t pv = 0, pg = 0, pnb = 0;
+  int64_t pp = 0;
+  for (int e0 = 0; e0 < count; e0 += NG) {
+    const int e = e0 + lane / GS;
+    const GEntry en = shfl_gentry(mine, e < count ? e : 0);
+    int32_t idx = -1;
+    if (e < count && en.d > 0 && en.F > 0) {
+      if ((en.flags & 1u) && en.d <= en.F) {
+        idx = gl < en.d ? gl : -1;
+      } else {
+        const int32_t res = group_select<KIND, GS>(G, en, gl, gbase, gmask);
+        idx = gl < en.F ? res : -1;
+      }
+    }
+    if (have) {
+      H.src[pp] = pv;
+      H.dst[pp] = pnb;
+      H.smp[pp] = pg;
+    }
+    have = idx >= 0;
+    if (have) {
+      pv = en.v;
+      pg = en.g;
+      pp = en.pos + gl;
+      pnb = Update(&hv, G.adj[en.o + idx], en.v);
+    }
+  }
+  if (have) {
+    H.src[pp] = pv;
+    H.dst[pp] = pnb;
+    H.smp[pp] = pg;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Lane-level selection (THREAD variant).  Valid for F <= kThreadFanCap and,
 // for integer biases, d <= kThreadIntCap (weights cached in LDS, transposed
 // per lane).  Taken set: a 64-bit mask when d <= 64, else a per-lane LDS list.
@@ -367,6 +608,9 @@ __device__ void expand_entry_thread(const GraphDev& G, const csaw_graph_view& hv
     Kf = G.cdf_idx;
     Tf = Cf[d - 1];
     if (Tf > 0.0f) mode = Kf ? kPickF32Idx : kPickF32;
+  } else if (KIND == kInt && G.int_cdf32[ck]) {
+    T = G.int_cdf32[ck][en.o + d - 1];
+    if (T > 0) mode = kPickInt32Idx;
   } else if (KIND == kInt && G.int_cdf[ck]) {
     Cg = G.int_cdf[ck] + en.o;
     Kg = G.int_idx[ck];
@@ -389,6 +633,8 @@ __device__ void expand_entry_thread(const GraphDev& G, const csaw_graph_view& hv
     const uint64_t x = mulhi64(t, T);
     if (mode == kPickInt) return (int32_t)upper_bound_dev<uint64_t>(Cg, d, x);
     if (mode == kPickIntIdx) return cdf_index_search<uint64_t>(G.int_cdf[ck], Kg, G.E, en.o, d, x);
+    if (mode == kPickInt32Idx)
+      return cdf_index_search<uint32_t>(G.int_cdf32[ck], G.int_idx32[ck], G.E, en.o, d, (uint32_t)x);
     uint64_t acc = 0;
     int32_t i = 0;
     for (; i < d - 1; ++i) {
@@ -402,7 +648,14 @@ __device__ void expand_entry_thread(const GraphDev& G, const csaw_graph_view& hv
   constexpr int G4 = 4;
   auto pick4 = [&](int j0, int nq, int32_t* out) {
     // (the index searches are instantiated only for their own kind)
-    if (KIND == kInt && mode == kPickIntIdx) {
+    if (KIND == kInt && mode == kPickInt32Idx) {
+      if constexpr (KIND == kInt) {
+        uint32_t xi[G4];
+#pragma unroll
+        for (int q = 0; q < G4; ++q) xi[q] = q < nq ? (uint32_t)mulhi64(philox_t64(draw(en, j0 + q, 0)), T) : 0u;
+        cdf_index_search_n<uint32_t, G4>(G.int_cdf32[ck], G.int_idx32[ck], G.E, en.o, d, xi, nq, out);
+      }
+    } else if (KIND == kInt && mode == kPickIntIdx) {
       if constexpr (KIND == kInt) {
         // 64-bit keys: two pairs in lockstep (16 keys of a search take 32 registers)
         uint64_t xi[G4];

@@ -32,6 +32,12 @@ __device__ __forceinline__ int32_t walk_pick(const GraphDev& G, const csaw_graph
   }
   if (KIND == kInt) {
     const int ck = bias == CSAW_BIAS_USER ? 1 : 0;
+    if (G.int_cdf32[ck] && G.int_idx32[ck]) {
+      const uint64_t T = G.int_cdf32[ck][o + d - 1];
+      if (T > 0)
+        return cdf_index_search<uint32_t>(G.int_cdf32[ck], G.int_idx32[ck], G.E, o, d, (uint32_t)mulhi64(t, T));
+      return (int32_t)mulhi64(t, (uint64_t)d);
+    }
     if (G.int_cdf[ck] && G.int_idx[ck]) {
       const uint64_t T = G.int_cdf[ck][o + d - 1];
       if (T > 0) return cdf_index_search<uint64_t>(G.int_cdf[ck], G.int_idx[ck], G.E, o, d, mulhi64(t, T));
@@ -79,7 +85,7 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk(GraphDev G, csaw_graph_vi
         const int32_t t = t0 + k;
         if (act && v >= 0) {
           const int64_t o = G.off[v];
-          const int32_t d = G.deg[v];
+          const int32_t d = (int32_t)(G.off[v + 1] - o);  // same line as off[v]
           if (d == 0) {
             v = -1;
           } else {
@@ -90,7 +96,8 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk(GraphDev G, csaw_graph_vi
             // walk entry 4, plus the CDF total and ceil(log2 d) probes (FP32 /
             // cached integer) or two passes over adj + degree (integer, built)
             const int l = ceil_log2_u32((uint32_t)d);
-            const bool cached = KIND == kInt && G.int_cdf[bias == CSAW_BIAS_USER ? 1 : 0];
+            const bool cached = KIND == kInt && (G.int_cdf[bias == CSAW_BIAS_USER ? 1 : 0] ||
+                                                 G.int_cdf32[bias == CSAW_BIAS_USER ? 1 : 0]);
             bytes += 20 + (KIND == kF32 ? 4 + 4 * l
                                         : (KIND == kInt ? (cached || d > kHubMin ? 8 + 8 * l : 16 * (unsigned long long)d)
                                                         : 0));

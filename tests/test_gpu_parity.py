@@ -197,6 +197,63 @@ def test_hipgraph_replays(C, O, g14):
         C.sample(g, specs[0]["seeds"], [3])
 
 
+def _cache_specs(seeds_of, fans, biases, dedups):
+    specs = []
+    for b in biases:
+        for f in fans:
+            for dd in dedups:
+                specs.append({"seeds": seeds_of(len(specs)), "fanouts": f, "bias": b, "dedup": dd,
+                              "request": len(specs)})
+    return specs
+
+
+def _check_cache_plans(C, O, g, og, specs, what):
+    """Every forced variant, the heuristic and random schedules, with the
+    CDF cache (index searches, sub-wave groups) against the oracle."""
+    want = [O.sample(og, s["seeds"], s["fanouts"], bias=s["bias"] if s["bias"] != 3 else O.BIAS_USER,
+                     dedup=s["dedup"], request=s["request"]) for s in specs]
+    runs = [(C.SCHED_FORCED, v) for v in range(C.NUM_VARIANTS)] + [(C.SCHED_HEURISTIC, 0), (C.SCHED_RANDOM, 0)]
+    for sched, v in runs:
+        p = C.Plan(g, specs, mode=C.MODE_BATCHED, sched=sched, variant=v, cdf_cache=True, explore_key=5)
+        p.run()
+        for i, s in enumerate(specs):
+            same(p.result(i), want[i], f"{what} sched {sched} variant {v} req {i} {s['fanouts']} b{s['bias']}")
+        p.close()
+
+
+def test_cdf_cache_variants_match_oracle(C, O, g14):
+    # fanouts spanning the group sizes 8 / 16 / 32 and the whole-wave path
+    g, og, *_ = g14
+    fans = [[25, 10], [15, 10, 5], [3, 2], [40, 2], [64]]
+    specs = _cache_specs(lambda i: g.seeds(150, request=i), fans, [0, 1, 2, 3], [1, 0])
+    _check_cache_plans(C, O, g, og, specs, "rmat14")
+
+
+def test_cdf_cache_collisions_and_hubs(C, O):
+    # d == F + 1 and small d force repeated draws (ordered slow path, cyclic
+    # fallback); the hub has degree 20000 (multi-level index search)
+    off, adj = star_with_hub(O, 20000, 4)
+    w = O.weights(len(adj), key=9)
+    g = C.Graph.from_csr(off, adj, w)
+    og = O.OracleGraph(off, adj, weights=w)
+    rng = np.random.default_rng(4)
+    pool = np.concatenate([np.zeros(40, np.int32), rng.integers(1, len(off) - 1, 200).astype(np.int32)])
+    fans = [[2, 2], [5, 3], [7, 9], [31, 4], [64, 1]]
+    specs = _cache_specs(lambda i: np.roll(pool, i)[:60], fans, [0, 1, 2, 3], [1])
+    _check_cache_plans(C, O, g, og, specs, "hub")
+
+
+def test_cdf_cache_edge_cases(C, O):
+    off = np.array([0, 3, 5, 5, 6, 7, 7, 9], np.int64)
+    adj = np.array([1, 2, 6, 0, 3, 4, 5, 2, 5], np.int32)
+    w = np.array([0.5, 0.25, 1.0, 1.0, 0.5, 1.0, 1.0, 0.0, 0.0], np.float32)
+    g = C.Graph.from_csr(off, adj, w)
+    og = O.OracleGraph(off, adj, weights=w)
+    seeds = np.array([0, 1, 2, 3, 4, 5, 6, 0, 6, 2], np.int32)
+    specs = _cache_specs(lambda i: seeds, [[1], [2], [3], [2, 2, 2], [64, 1]], [0, 1, 2], [0, 1])
+    _check_cache_plans(C, O, g, og, specs, "edge")
+
+
 @pytest.mark.parametrize("bias", [0, 1, 2])
 def test_walk_matches_oracle(C, O, g14, bias):
     g, og, *_ = g14
