@@ -263,6 +263,7 @@ __global__ __launch_bounds__(256) void k_expand_plain(GraphDev G, csaw_graph_vie
                                                       const int32_t* __restrict__ gs2req, HopDev H, int gs) {
   __shared__ uint64_t cdf[KIND == kInt ? 4 * kWaveIntCap : 1];
   __shared__ uint32_t bms[4 * (kWaveBitmapCap / 32)];
+  __shared__ uint32_t glds[KIND == kF32 ? 4 * kGroupLdsWords : 1];
   const int64_t n = *H.n_ptr;
   const int wid = threadIdx.x / kWave;
   const int64_t nw = (int64_t)gridDim.x * 4;
@@ -271,19 +272,20 @@ __global__ __launch_bounds__(256) void k_expand_plain(GraphDev G, csaw_graph_vie
   // when it is large
   const int64_t per = KIND != kInt && gs > 0 ? kWave / gs : 1;
   const int64_t bs = min((int64_t)kWave, max(per, (n + nw - 1) / nw));
+  uint32_t* gl = KIND == kF32 ? glds + wid * kGroupLdsWords : nullptr;
   for (int64_t e0 = (blockIdx.x * 4 + wid) * bs; e0 < n; e0 += nw * bs) {
     const int cnt = (int)min(bs, n - e0);
     if constexpr (KIND != kInt) {
       if (gs == 8) {
-        group_expand_batch<KIND, 8>(G, hv, R, gs2req, H, nullptr, e0, cnt);
+        group_expand_batch<KIND, 8>(G, hv, R, gs2req, H, nullptr, e0, cnt, gl);
         continue;
       }
       if (gs == 16) {
-        group_expand_batch<KIND, 16>(G, hv, R, gs2req, H, nullptr, e0, cnt);
+        group_expand_batch<KIND, 16>(G, hv, R, gs2req, H, nullptr, e0, cnt, gl);
         continue;
       }
       if (gs == 32) {
-        group_expand_batch<KIND, 32>(G, hv, R, gs2req, H, nullptr, e0, cnt);
+        group_expand_batch<KIND, 32>(G, hv, R, gs2req, H, nullptr, e0, cnt, gl);
         continue;
       }
     }
@@ -505,6 +507,10 @@ __global__ __launch_bounds__(256) void k_expand_wave(GraphDev G, csaw_graph_view
   // chunks whose fanouts fit a sub-wave group (and need no CDF build) run
   // 64 / GS entries at a time
   const bool groups_ok = KIND != kInt || A.int_cache;
+  // small-degree CDF slots: the integer kind reuses its (then idle) CDF region
+  __shared__ uint32_t glds[KIND == kF32 ? 4 * kGroupLdsWords : 1];
+  uint32_t* gl = KIND == kF32 ? glds + wid * kGroupLdsWords
+                              : (KIND == kInt ? reinterpret_cast<uint32_t*>(cdf + wid * kWaveIntCap) : nullptr);
   for (;;) {
     const ChunkCursor cc = next_chunk_wave(A, q);
     if (!cc.valid) break;
@@ -514,9 +520,9 @@ __global__ __launch_bounds__(256) void k_expand_wave(GraphDev G, csaw_graph_view
     for (int k0 = 0; k0 < c.len; k0 += kWave) {
       const int64_t first = (int64_t)c.start + k0;
       const int cnt = min(kWave, (int)c.len - k0);
-      if (groups_ok && gs == 8) group_expand_batch<KIND, 8>(G, hv, R, gs2req, H, H.order, first, cnt);
-      else if (groups_ok && gs == 16) group_expand_batch<KIND, 16>(G, hv, R, gs2req, H, H.order, first, cnt);
-      else if (groups_ok && gs == 32) group_expand_batch<KIND, 32>(G, hv, R, gs2req, H, H.order, first, cnt);
+      if (groups_ok && gs == 8) group_expand_batch<KIND, 8>(G, hv, R, gs2req, H, H.order, first, cnt, gl);
+      else if (groups_ok && gs == 16) group_expand_batch<KIND, 16>(G, hv, R, gs2req, H, H.order, first, cnt, gl);
+      else if (groups_ok && gs == 32) group_expand_batch<KIND, 32>(G, hv, R, gs2req, H, H.order, first, cnt, gl);
       else
         wave_expand_batch<KIND, true>(G, hv, R, gs2req, H, H.order, first, cnt,
                                       KIND == kInt ? cdf + wid * kWaveIntCap : nullptr,

@@ -473,13 +473,46 @@ __device__ __forceinline__ bool group_taken(int32_t ci, int32_t res, int jj, int
   return hit;
 }
 
+// Small-degree entries (d <= kGroupSmallCap) with 32-bit CDF keys: the group
+// copies the entry's CDF into its LDS slot with one coalesced load and every
+// draw is an upper_bound in LDS, with no further global access.
+constexpr int kGroupSmallCap = 64;
+
+template <int KIND>
+__device__ __forceinline__ int32_t gpick_small(const GEntry& en, const uint32_t* lc, const U4& r) {
+  if constexpr (KIND == kF32) {
+    const float x = __fmul_rn(philox_u24(r), __uint_as_float((uint32_t)en.T));
+    return (int32_t)upper_bound_dev<float>(reinterpret_cast<const float*>(lc), en.d, x);
+  } else {
+    const uint32_t x = (uint32_t)mulhi64(philox_t64(r), en.T);
+    return (int32_t)upper_bound_dev<uint32_t>(lc, en.d, x);
+  }
+}
+
+// lc: the group's LDS slot of kGroupSmallCap words (null: no small path)
 template <int KIND, int GS>
-__device__ int32_t group_select(const GraphDev& G, const GEntry& en, int gl, int gbase, uint64_t gmask) {
+__device__ int32_t group_select(const GraphDev& G, const GEntry& en, int gl, int gbase, uint64_t gmask,
+                                uint32_t* lc) {
   const int F = en.F;
   const int32_t d = en.d;
   const bool act = gl < F;
-  int32_t cand = -1;
-  if (act) cand = gpick<KIND>(G, en, gdraw(en, gl, 0));
+  const uint32_t mode = (en.flags >> 4) & 15u;
+  const bool small = KIND != kUniform && lc && d <= kGroupSmallCap &&
+                     (mode == kPickInt32Idx || mode == kPickF32 || mode == kPickF32Idx);
+  if (small) {
+    const uint32_t* C = KIND == kF32 ? reinterpret_cast<const uint32_t*>(G.cdf)
+                                     : ((en.flags >> 1) & 1u ? G.int_cdf32[1] : G.int_cdf32[0]);
+    for (int i = gl; i < d; i += GS) lc[i] = C[en.o + i];
+    wave_fence();
+  }
+  // one draw (slot j, attempt a); only `valid` lanes' results are used
+  auto pick = [&](int j, int a, bool valid) -> int32_t {
+    if (!valid) return 0;
+    if (small) return gpick_small<KIND>(en, lc, gdraw(en, j, a));
+    return gpick<KIND>(G, en, gdraw(en, j, a));
+  };
+  int32_t cand = pick(gl, 0, act);
+  if (!act) cand = -1;
   if (!(en.flags & 1u)) return cand;
   bool dup = false;
   for (int i = 0; i + 1 < F; ++i) {
@@ -496,8 +529,8 @@ __device__ int32_t group_select(const GraphDev& G, const GEntry& en, int gl, int
       int32_t chosen = -1, last = c;
       for (int a0 = 1; a0 < kReselect; a0 += GS) {
         const int a = a0 + gl;
-        int32_t ci = 0;
-        if (a < kReselect) ci = gpick<KIND>(G, en, gdraw(en, jj, a));
+        int32_t ci = pick(jj, a, a < kReselect);
+        if (a >= kReselect) ci = 0;
         const bool tk = group_taken(ci, res, jj, gbase);
         const uint64_t m = ballot((a < kReselect) && !tk) & gmask;
         if (m) {
@@ -527,14 +560,18 @@ __device__ int32_t group_select(const GraphDev& G, const GEntry& en, int gl, int
 
 // Expands `count` (<= 64) entries, all with F <= GS, in groups of GS lanes.
 // Needs no per-entry CDF build: uniform, FP32, or integer biases with the
-// CDF cache.
+// CDF cache.  lds: the wave's kGroupLdsWords words for small-degree CDFs
+// (null: searches only).
+constexpr int kGroupLdsWords = (kWave / 8) * kGroupSmallCap;
+
 template <int KIND, int GS>
 __device__ void group_expand_batch(const GraphDev& G, const csaw_graph_view& hv, const ReqDev* __restrict__ R,
                                    const int32_t* __restrict__ gs2req, const HopDev& H,
-                                   const int32_t* __restrict__ order, int64_t first, int count) {
+                                   const int32_t* __restrict__ order, int64_t first, int count, uint32_t* lds) {
   constexpr int NG = kWave / GS;
   const int lane = lane_id();
   const int gl = lane & (GS - 1), gbase = lane & ~(GS - 1);
+  uint32_t* lc = lds ? lds + (gbase / GS) * kGroupSmallCap : nullptr;
   const uint64_t gmask = (GS >= kWave ? ~0ull : ((1ull << GS) - 1ull)) << gbase;
   GEntry mine;
   mine.d = 0;
@@ -555,7 +592,7 @@ __device__ void group_expand_batch(const GraphDev& G, const csaw_graph_view& hv,
       if ((en.flags & 1u) && en.d <= en.F) {
         idx = gl < en.d ? gl : -1;
       } else {
-        const int32_t res = group_select<KIND, GS>(G, en, gl, gbase, gmask);
+        const int32_t res = group_select<KIND, GS>(G, en, gl, gbase, gmask, lc);
         idx = gl < en.F ? res : -1;
       }
     }
