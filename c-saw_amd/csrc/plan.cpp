@@ -285,7 +285,7 @@ static int run_unit(csaw_plan* p, Unit& u, hipStream_t st, TimingHook* th) {
       int fmax = 0;
       for (int ri : u.reqs)
         if (p->reqs[ri].depth > h) fmax = std::max(fmax, p->reqs[ri].fanouts[h]);
-      const int gs = fmax <= 8 ? 8 : (fmax <= 16 ? 16 : (fmax <= 32 ? 32 : 0));
+      const int gs = fmax <= 8 ? 8 : (fmax <= 16 ? 16 : (fmax <= 32 ? 32 : 64));
       if (th) th->begin(kKernPlain, st);
       CSAW_TRY(launch_expand_plain(L, kind, G, hv, u.d_req, u.d_gs2req, H, u.nmax[h], gs));
       if (th) th->end(kKernPlain, st);

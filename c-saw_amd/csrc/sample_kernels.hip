@@ -256,8 +256,8 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, int64_t n) {
 }
 
 // ----------------------------------------------------------- expand plain
-// gs: sub-wave group size for the hop's fanout (8, 16 or 32; 0 = whole waves),
-// used for the kinds that need no per-entry CDF build (uniform, FP32).
+// gs: group size for the hop's fanout (8, 16, 32 or 64; 0 = the whole-wave
+// path), used for the kinds that need no per-entry CDF build (uniform, FP32).
 template <int KIND>
 __global__ __launch_bounds__(256) void k_expand_plain(GraphDev G, csaw_graph_view hv, const ReqDev* __restrict__ R,
                                                       const int32_t* __restrict__ gs2req, HopDev H, int gs) {
@@ -276,16 +276,8 @@ __global__ __launch_bounds__(256) void k_expand_plain(GraphDev G, csaw_graph_vie
   for (int64_t e0 = (blockIdx.x * 4 + wid) * bs; e0 < n; e0 += nw * bs) {
     const int cnt = (int)min(bs, n - e0);
     if constexpr (KIND != kInt) {
-      if (gs == 8) {
-        group_expand_batch<KIND, 8>(G, hv, R, gs2req, H, nullptr, e0, cnt, gl);
-        continue;
-      }
-      if (gs == 16) {
-        group_expand_batch<KIND, 16>(G, hv, R, gs2req, H, nullptr, e0, cnt, gl);
-        continue;
-      }
-      if (gs == 32) {
-        group_expand_batch<KIND, 32>(G, hv, R, gs2req, H, nullptr, e0, cnt, gl);
+      if (gs > 0) {
+        group_expand_batch<KIND>(G, hv, R, gs2req, H, nullptr, e0, cnt, gs, gl);
         continue;
       }
     }
@@ -495,11 +487,17 @@ __global__ __launch_bounds__(256) void k_expand_thread(GraphDev G, csaw_graph_vi
   }
 }
 
-template <int KIND, bool BITMAP>
+// G32: group-only build for views whose CDF keys are all 32-bit (uniform,
+// FP32, integer biases with uint32 caches): every chunk runs in sub-wave
+// groups (GS = 64 for fanouts above 32) and the whole-wave path, LDS CDF
+// builds and uint64 searches are compiled out.
+template <int KIND, bool BITMAP, bool G32>
 __global__ __launch_bounds__(256) void k_expand_wave(GraphDev G, csaw_graph_view hv, const ReqDev* __restrict__ R,
                                                      const int32_t* __restrict__ gs2req, HopDev H, ExpandArgs A) {
-  __shared__ uint64_t cdf[KIND == kInt ? 4 * kWaveIntCap : 1];
-  __shared__ uint32_t bms[BITMAP ? 4 * (kWaveBitmapCap / 32) : 1];
+  __shared__ uint64_t cdf[!G32 && KIND == kInt ? 4 * kWaveIntCap : 1];
+  __shared__ uint32_t bms[!G32 && BITMAP ? 4 * (kWaveBitmapCap / 32) : 1];
+  // small-degree CDF slots (the generic integer kernel reuses its CDF region)
+  __shared__ uint32_t glds[KIND == kF32 || (G32 && KIND == kInt) ? 4 * kGroupLdsWords : 1];
   const int var = BITMAP ? CSAW_VAR_WAVE_BITMAP : CSAW_VAR_WAVE_HASH;
   const int q = var * 3 + KIND;
   const int wid = threadIdx.x / kWave;
@@ -507,26 +505,27 @@ __global__ __launch_bounds__(256) void k_expand_wave(GraphDev G, csaw_graph_view
   // chunks whose fanouts fit a sub-wave group (and need no CDF build) run
   // 64 / GS entries at a time
   const bool groups_ok = KIND != kInt || A.int_cache;
-  // small-degree CDF slots: the integer kind reuses its (then idle) CDF region
-  __shared__ uint32_t glds[KIND == kF32 ? 4 * kGroupLdsWords : 1];
-  uint32_t* gl = KIND == kF32 ? glds + wid * kGroupLdsWords
-                              : (KIND == kInt ? reinterpret_cast<uint32_t*>(cdf + wid * kWaveIntCap) : nullptr);
+  uint32_t* gl = nullptr;
+  if (KIND == kF32 || (G32 && KIND == kInt)) gl = glds + wid * kGroupLdsWords;
+  else if (KIND == kInt) gl = reinterpret_cast<uint32_t*>(cdf + wid * kWaveIntCap);
   for (;;) {
     const ChunkCursor cc = next_chunk_wave(A, q);
     if (!cc.valid) break;
     const uint64_t t0 = wall_clock64();
     const Chunk c = H.chunks[cc.ci];
-    const int gs = c.max_f <= 8 ? 8 : (c.max_f <= 16 ? 16 : (c.max_f <= 32 ? 32 : kWave));
+    const int gs = group_size_for(c.max_f);
     for (int k0 = 0; k0 < c.len; k0 += kWave) {
       const int64_t first = (int64_t)c.start + k0;
       const int cnt = min(kWave, (int)c.len - k0);
-      if (groups_ok && gs == 8) group_expand_batch<KIND, 8>(G, hv, R, gs2req, H, H.order, first, cnt, gl);
-      else if (groups_ok && gs == 16) group_expand_batch<KIND, 16>(G, hv, R, gs2req, H, H.order, first, cnt, gl);
-      else if (groups_ok && gs == 32) group_expand_batch<KIND, 32>(G, hv, R, gs2req, H, H.order, first, cnt, gl);
-      else
-        wave_expand_batch<KIND, true>(G, hv, R, gs2req, H, H.order, first, cnt,
-                                      KIND == kInt ? cdf + wid * kWaveIntCap : nullptr,
-                                      BITMAP ? bms + wid * (kWaveBitmapCap / 32) : nullptr);
+      if constexpr (G32) {
+        group_expand_batch<KIND, kKeys32>(G, hv, R, gs2req, H, H.order, first, cnt, gs, gl);
+      } else {
+        if (groups_ok && gs < kWave) group_expand_batch<KIND>(G, hv, R, gs2req, H, H.order, first, cnt, gs, gl);
+        else
+          wave_expand_batch<KIND, true>(G, hv, R, gs2req, H, H.order, first, cnt,
+                                        KIND == kInt ? cdf + wid * kWaveIntCap : nullptr,
+                                        BITMAP ? bms + wid * (kWaveBitmapCap / 32) : nullptr);
+      }
     }
     log_chunk_wave(A, H, cc.ci, var, t0);
   }
@@ -545,11 +544,25 @@ __global__ __launch_bounds__(256) void k_expand_block(GraphDev G, csaw_graph_vie
   const int var = BITMAP ? CSAW_VAR_BLOCK_BITMAP : CSAW_VAR_BLOCK_HASH;
   const int q = var * 3 + KIND;
   const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
+  // Entries that need no CDF build (the CDF cache is on) have nothing for a
+  // workgroup to share: its four waves then split the chunk into 64-entry
+  // batches expanded in sub-wave groups.
+  const bool groups_ok = KIND != kInt || A.int_cache;
+  uint32_t* gl = KIND == kInt ? reinterpret_cast<uint32_t*>(cdf) + wid * kGroupLdsWords : nullptr;
   for (;;) {
     const ChunkCursor cc = next_chunk(A, q, &slot);
     if (!cc.valid) break;
     const uint64_t t0 = wall_clock64();
     const Chunk c = H.chunks[cc.ci];
+    if (groups_ok) {
+      const int gs = group_size_for(c.max_f);
+      for (int k0 = wid * kWave; k0 < c.len; k0 += 4 * kWave)
+        group_expand_batch<KIND>(G, hv, R, gs2req, H, H.order, (int64_t)c.start + k0, min(kWave, (int)c.len - k0), gs,
+                                 gl);
+      __syncthreads();
+      log_chunk(A, H, cc.ci, var, t0);
+      continue;
+    }
     for (int k = 0; k < c.len; ++k) {
       const Entry en = load_entry(G, R, gs2req, H, H.order[c.start + k]);
       if (en.d == 0 || en.F == 0) continue;
@@ -594,17 +607,18 @@ __global__ __launch_bounds__(256) void k_expand_block(GraphDev G, csaw_graph_vie
   }
 }
 
-// copy-all chunks (d <= F without replacement): one lane per entry
+// copy-all chunks (d <= F without replacement): sub-wave groups, lane i of a
+// group copying neighbour i (coalesced reads and writes)
 __global__ __launch_bounds__(256) void k_expand_copy(GraphDev G, csaw_graph_view hv, const ReqDev* __restrict__ R,
                                                      const int32_t* __restrict__ gs2req, HopDev H, ExpandArgs A) {
   for (;;) {
     const ChunkCursor cc = next_chunk_wave(A, kQueueCopy);
     if (!cc.valid) break;
     const Chunk c = H.chunks[cc.ci];
-    for (int k = lane_id(); k < c.len; k += kWave) {
-      const Entry en = load_entry(G, R, gs2req, H, H.order[c.start + k]);
-      for (int32_t i = 0; i < en.d; ++i) write_edge(G, hv, H, en, en.pos + i, i);
-    }
+    const int gs = group_size_for(c.max_f);
+    for (int k0 = 0; k0 < c.len; k0 += kWave)
+      group_expand_batch<kUniform, kKeys32>(G, hv, R, gs2req, H, H.order, (int64_t)c.start + k0,
+                                            min(kWave, (int)c.len - k0), gs, nullptr);
   }
 }
 
@@ -681,20 +695,27 @@ int launch_expand_chunks(const LaunchCfg& L, const GraphDev& G, const csaw_graph
     if (!kinds_present[kind]) continue;
     if (kind == kUniform) {
       CSAW_TRY(go(kKernThread, k_expand_thread<kUniform>, L.thread_per_cu));
-      CSAW_TRY(go(kKernWave, k_expand_wave<kUniform, true>, L.expand_per_cu));
-      CSAW_TRY(go(kKernWave, k_expand_wave<kUniform, false>, L.expand_per_cu));
+      CSAW_TRY(go(kKernWave, k_expand_wave<kUniform, true, true>, L.expand_per_cu));
+      CSAW_TRY(go(kKernWave, k_expand_wave<kUniform, false, true>, L.expand_per_cu));
       CSAW_TRY(go(kKernBlock, k_expand_block<kUniform, true>, L.block_per_cu));
       CSAW_TRY(go(kKernBlock, k_expand_block<kUniform, false>, L.block_per_cu));
     } else if (kind == kInt) {
+      // group-only build when every integer cache of the view is uint32
+      const bool g32 = A.int_cache && !G.int_cdf[0] && !G.int_cdf[1];
       CSAW_TRY(go(kKernThread, k_expand_thread<kInt>, L.thread_per_cu));
-      CSAW_TRY(go(kKernWave, k_expand_wave<kInt, true>, L.expand_per_cu));
-      CSAW_TRY(go(kKernWave, k_expand_wave<kInt, false>, L.expand_per_cu));
+      if (g32) {
+        CSAW_TRY(go(kKernWave, k_expand_wave<kInt, true, true>, L.expand_per_cu));
+        CSAW_TRY(go(kKernWave, k_expand_wave<kInt, false, true>, L.expand_per_cu));
+      } else {
+        CSAW_TRY(go(kKernWave, k_expand_wave<kInt, true, false>, L.expand_per_cu));
+        CSAW_TRY(go(kKernWave, k_expand_wave<kInt, false, false>, L.expand_per_cu));
+      }
       CSAW_TRY(go(kKernBlock, k_expand_block<kInt, true>, L.block_per_cu));
       CSAW_TRY(go(kKernBlock, k_expand_block<kInt, false>, L.block_per_cu));
     } else {
       CSAW_TRY(go(kKernThread, k_expand_thread<kF32>, L.thread_per_cu));
-      CSAW_TRY(go(kKernWave, k_expand_wave<kF32, true>, L.expand_per_cu));
-      CSAW_TRY(go(kKernWave, k_expand_wave<kF32, false>, L.expand_per_cu));
+      CSAW_TRY(go(kKernWave, k_expand_wave<kF32, true, true>, L.expand_per_cu));
+      CSAW_TRY(go(kKernWave, k_expand_wave<kF32, false, true>, L.expand_per_cu));
       CSAW_TRY(go(kKernBlock, k_expand_block<kF32, true>, L.block_per_cu));
       CSAW_TRY(go(kKernBlock, k_expand_block<kF32, false>, L.block_per_cu));
     }

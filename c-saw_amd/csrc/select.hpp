@@ -420,7 +420,12 @@ __device__ __forceinline__ U4 gdraw(const GEntry& en, int j, int a) {
   return philox4x32_10(((uint32_t)j << 16) | (uint32_t)a, en.c1, en.c2, en.c3, en.k0, en.k1);
 }
 
-template <int KIND>
+// Key widths a group kernel is compiled for: any, or 32-bit keys only (every
+// integer cache of the view is uint32; the uint64 searches are compiled out,
+// which frees their registers).
+enum GroupKeys { kKeysAny = 0, kKeys32 = 1 };
+
+template <int KIND, int KEYS = kKeysAny>
 __device__ __forceinline__ int32_t gpick(const GraphDev& G, const GEntry& en, const U4& r) {
   const uint32_t mode = (en.flags >> 4) & 15u;
   const uint64_t t = philox_t64(r);
@@ -432,7 +437,7 @@ __device__ __forceinline__ int32_t gpick(const GraphDev& G, const GEntry& en, co
   }
   const bool user = (en.flags >> 1) & 1u;
   const uint64_t x = mulhi64(t, en.T);
-  if (mode == kPickInt32Idx)
+  if (KEYS == kKeys32 || mode == kPickInt32Idx)
     return cdf_index_search<uint32_t>(user ? G.int_cdf32[1] : G.int_cdf32[0], user ? G.int_idx32[1] : G.int_idx32[0],
                                       G.E, en.o, en.d, (uint32_t)x);
   const uint64_t* C = user ? G.int_cdf[1] : G.int_cdf[0];
@@ -490,8 +495,9 @@ __device__ __forceinline__ int32_t gpick_small(const GEntry& en, const uint32_t*
 }
 
 // lc: the group's LDS slot of kGroupSmallCap words (null: no small path)
-template <int KIND, int GS>
-__device__ int32_t group_select(const GraphDev& G, const GEntry& en, int gl, int gbase, uint64_t gmask,
+// gs: group size (8, 16, 32 or 64), a run-time value so one build serves all
+template <int KIND, int KEYS>
+__device__ int32_t group_select(const GraphDev& G, const GEntry& en, int gs, int gl, int gbase, uint64_t gmask,
                                 uint32_t* lc) {
   const int F = en.F;
   const int32_t d = en.d;
@@ -502,14 +508,14 @@ __device__ int32_t group_select(const GraphDev& G, const GEntry& en, int gl, int
   if (small) {
     const uint32_t* C = KIND == kF32 ? reinterpret_cast<const uint32_t*>(G.cdf)
                                      : ((en.flags >> 1) & 1u ? G.int_cdf32[1] : G.int_cdf32[0]);
-    for (int i = gl; i < d; i += GS) lc[i] = C[en.o + i];
+    for (int i = gl; i < d; i += gs) lc[i] = C[en.o + i];
     wave_fence();
   }
   // one draw (slot j, attempt a); only `valid` lanes' results are used
   auto pick = [&](int j, int a, bool valid) -> int32_t {
     if (!valid) return 0;
     if (small) return gpick_small<KIND>(en, lc, gdraw(en, j, a));
-    return gpick<KIND>(G, en, gdraw(en, j, a));
+    return gpick<KIND, KEYS>(G, en, gdraw(en, j, a));
   };
   int32_t cand = pick(gl, 0, act);
   if (!act) cand = -1;
@@ -527,7 +533,7 @@ __device__ int32_t group_select(const GraphDev& G, const GEntry& en, int gl, int
     int32_t c = shfl_abs(cand, gbase + jj);
     if ((ballot(res == c) & gmask) != 0ull) {
       int32_t chosen = -1, last = c;
-      for (int a0 = 1; a0 < kReselect; a0 += GS) {
+      for (int a0 = 1; a0 < kReselect; a0 += gs) {
         const int a = a0 + gl;
         int32_t ci = pick(jj, a, a < kReselect);
         if (a >= kReselect) ci = 0;
@@ -538,10 +544,10 @@ __device__ int32_t group_select(const GraphDev& G, const GEntry& en, int gl, int
           break;
         }
         const int ll = kReselect - 1 - a0;
-        if (ll < GS) last = shfl_abs(ci, gbase + ll);
+        if (ll < gs) last = shfl_abs(ci, gbase + ll);
       }
       if (chosen < 0) {
-        for (int q = 0; q < d; q += GS) {
+        for (int q = 0; q < d; q += gs) {
           const int32_t i = (int32_t)(((int64_t)last + 1 + q + gl) % d);
           const bool tk = group_taken(i, res, jj, gbase);
           const uint64_t m = ballot((q + gl < d) && !tk) & gmask;
@@ -558,21 +564,27 @@ __device__ int32_t group_select(const GraphDev& G, const GEntry& en, int gl, int
   return res;
 }
 
-// Expands `count` (<= 64) entries, all with F <= GS, in groups of GS lanes.
-// Needs no per-entry CDF build: uniform, FP32, or integer biases with the
-// CDF cache.  lds: the wave's kGroupLdsWords words for small-degree CDFs
-// (null: searches only).
+// Expands `count` (<= 64) entries, all with F <= gs, in groups of gs lanes
+// (gs = 8, 16, 32 or 64).  Needs no per-entry CDF build: uniform, FP32, or
+// integer biases with the CDF cache.  lds: the wave's kGroupLdsWords words for
+// small-degree CDFs (null: searches only).
 constexpr int kGroupLdsWords = (kWave / 8) * kGroupSmallCap;
 
-template <int KIND, int GS>
+__device__ __forceinline__ int group_size_for(int max_f) {
+  return max_f <= 8 ? 8 : (max_f <= 16 ? 16 : (max_f <= 32 ? 32 : kWave));
+}
+
+template <int KIND, int KEYS = kKeysAny>
 __device__ void group_expand_batch(const GraphDev& G, const csaw_graph_view& hv, const ReqDev* __restrict__ R,
                                    const int32_t* __restrict__ gs2req, const HopDev& H,
-                                   const int32_t* __restrict__ order, int64_t first, int count, uint32_t* lds) {
-  constexpr int NG = kWave / GS;
+                                   const int32_t* __restrict__ order, int64_t first, int count, int gs,
+                                   uint32_t* lds) {
+  const int lg = __builtin_ctz((unsigned)gs);
+  const int NG = kWave >> lg;
   const int lane = lane_id();
-  const int gl = lane & (GS - 1), gbase = lane & ~(GS - 1);
-  uint32_t* lc = lds ? lds + (gbase / GS) * kGroupSmallCap : nullptr;
-  const uint64_t gmask = (GS >= kWave ? ~0ull : ((1ull << GS) - 1ull)) << gbase;
+  const int gl = lane & (gs - 1), gbase = lane & ~(gs - 1);
+  uint32_t* lc = lds ? lds + (gbase >> lg) * kGroupSmallCap : nullptr;
+  const uint64_t gmask = (gs >= kWave ? ~0ull : ((1ull << gs) - 1ull)) << gbase;
   GEntry mine;
   mine.d = 0;
   mine.F = 0;
@@ -585,14 +597,14 @@ __device__ void group_expand_batch(const GraphDev& G, const csaw_graph_view& hv,
   int32_t pv = 0, pg = 0, pnb = 0;
   int64_t pp = 0;
   for (int e0 = 0; e0 < count; e0 += NG) {
-    const int e = e0 + lane / GS;
+    const int e = e0 + (lane >> lg);
     const GEntry en = shfl_gentry(mine, e < count ? e : 0);
     int32_t idx = -1;
     if (e < count && en.d > 0 && en.F > 0) {
       if ((en.flags & 1u) && en.d <= en.F) {
         idx = gl < en.d ? gl : -1;
       } else {
-        const int32_t res = group_select<KIND, GS>(G, en, gl, gbase, gmask, lc);
+        const int32_t res = group_select<KIND, KEYS>(G, en, gs, gl, gbase, gmask, lc);
         idx = gl < en.F ? res : -1;
       }
     }
@@ -640,7 +652,25 @@ __device__ void expand_entry_thread(const GraphDev& G, const csaw_graph_view& hv
   const float* Kf = nullptr;
   const uint64_t* Kg = nullptr;
   const int ck = en.bias == CSAW_BIAS_USER ? 1 : 0;
-  if (KIND == kF32) {
+  // Small degrees with a 32-bit CDF in memory: the lane copies it into its LDS
+  // column once and every draw (repeated ones included) searches LDS.
+  constexpr int kPickLdsCdf = 7;
+  const uint32_t* small = nullptr;
+  if (d <= kThreadIntCap) {
+    if (KIND == kF32) small = reinterpret_cast<const uint32_t*>(G.cdf);
+    else if (KIND == kInt) small = G.int_cdf32[ck];
+  }
+  if (small) {
+    for (int32_t i = 0; i < d; ++i) lds[i * stride] = small[en.o + i];
+    const uint32_t last = lds[(d - 1) * stride];
+    if (KIND == kF32) {
+      Tf = __uint_as_float(last);
+      if (Tf > 0.0f) mode = kPickLdsCdf;
+    } else {
+      T = last;
+      if (T > 0) mode = kPickLdsCdf;
+    }
+  } else if (KIND == kF32) {
     Cf = G.cdf + en.o;
     Kf = G.cdf_idx;
     Tf = Cf[d - 1];
@@ -664,6 +694,19 @@ __device__ void expand_entry_thread(const GraphDev& G, const csaw_graph_view& hv
   auto pick = [&](const U4& o) -> int32_t {
     const uint64_t t = philox_t64(o);
     if (mode == kPickUniform) return (int32_t)mulhi64(t, (uint64_t)d);
+    if (mode == kPickLdsCdf) {
+      // upper_bound over the lane's LDS column
+      const float xf = KIND == kF32 ? __fmul_rn(philox_u24(o), Tf) : 0.f;
+      const uint32_t xu = KIND == kF32 ? 0u : (uint32_t)mulhi64(t, T);
+      int32_t lo = 0, hi = d;
+      while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        const uint32_t v = lds[mid * stride];
+        const bool gt = KIND == kF32 ? __uint_as_float(v) > xf : v > xu;
+        if (gt) hi = mid; else lo = mid + 1;
+      }
+      return lo;
+    }
     if (mode == kPickF32) return (int32_t)upper_bound_dev<float>(Cf, d, __fmul_rn(philox_u24(o), Tf));
     if (mode == kPickF32Idx)
       return cdf_index_search<float>(G.cdf, Kf, G.E, en.o, d, __fmul_rn(philox_u24(o), Tf));
