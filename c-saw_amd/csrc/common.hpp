@@ -48,7 +48,12 @@ constexpr int kWaveBitmapCap = 16384;// WAVE bitmap taken-set, bits
 constexpr int kBlockBitmapCap = 65536;
 constexpr int kNumBuckets = 128;     // 0 = copy-all; 1 + bias*32 + ceil(log2 d)
 constexpr int kBucketCopy = 0;
-constexpr int kChunksPerBlock = 48;  // chunk descriptor capacity per count block
+// Chunk descriptors of a hop: a block's buckets (at most 1 + 3 kinds x 8
+// degree classes non-empty) split into pieces of cmax entries give at most
+// 25 + 1024 / cmax per block, and cmax >= n / (expand waves), so the hop has
+// at most 26 * blocks + waves chunks.
+constexpr int kChunksPerBlock = 26;
+constexpr int kChunkMin = 8;         // smallest chunk piece (buckets split for small hops)
 
 // Bias kinds inside kernels: integer biases share one code path.
 enum BiasKind : int { kUniform = 0, kInt = 1, kF32 = 2 };
@@ -113,7 +118,20 @@ struct HopDev {
   int64_t* eo;          // per-sample edge offsets of this hop
   uint32_t* flags;      // bit 0: chunk overflow
   unsigned long long* bytes;  // algorithmic bytes per kernel kind (null: not counted)
+  int32_t expand_waves; // waves of one expand launch (sizes chunks; 0: whole buckets)
 };
+
+// Entries per chunk for a hop of n frontier entries: whole 1024-entry buckets
+// when the hop gives every expand wave several of them, smaller pieces (down
+// to kChunkMin) when it is small, so that the work still spreads over the
+// waves.
+__host__ __device__ __forceinline__ uint32_t chunk_max_for(int64_t n, int32_t waves) {
+  if (waves <= 0) return (uint32_t)kBlockEntries;
+  const int64_t per = (n + waves - 1) / waves;
+  uint32_t c = (uint32_t)kChunkMin;
+  while (c < (uint32_t)kBlockEntries && (int64_t)c < per) c <<= 1;
+  return c;
+}
 
 __host__ __device__ __forceinline__ int ceil_log2_u32(uint32_t d) {
   // smallest c with 2^c >= d (d >= 1)

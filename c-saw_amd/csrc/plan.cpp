@@ -227,6 +227,7 @@ static HopDev make_hop(csaw_plan* p, Unit& u, int h) {
   H.eo = u.d_eo[h];
   H.flags = p->counters + kCtrFlags;
   H.bytes = p->opts.timing ? p->bytes_acc : nullptr;
+  H.expand_waves = p->L.num_cu * p->L.expand_per_cu * 4;  // 4 waves per 256-lane workgroup
   return H;
 }
 
@@ -369,7 +370,8 @@ extern "C" int csaw_plan_create(csaw_graph* g, const csaw_request* reqs, int32_t
     CSAW_HIP(hipMemset(p->bytes_acc, 0, sizeof(unsigned long long) * kNumKernKinds));
     if (o.mode == CSAW_MODE_BATCHED) {
       CSAW_HIP(hipMalloc(&p->order, sizeof(int32_t) * nblk * kBlockEntries));
-      p->chunk_cap = (uint32_t)std::min<int64_t>(nblk * kChunksPerBlock, 0xFFFFFFF0LL);
+      const int64_t waves = (int64_t)p->L.num_cu * p->L.expand_per_cu * 4;
+      p->chunk_cap = (uint32_t)std::min<int64_t>(nblk * kChunksPerBlock + waves, 0xFFFFFFF0LL);
       CSAW_HIP(hipMalloc(&p->chunks, sizeof(Chunk) * p->chunk_cap));
       p->queue_cap = p->chunk_cap;
       CSAW_HIP(hipMalloc(&p->queues, sizeof(uint32_t) * (int64_t)kNumQueues * p->queue_cap));

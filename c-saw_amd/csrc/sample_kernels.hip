@@ -113,33 +113,40 @@ __global__ __launch_bounds__(256) void k_count(GraphDev G, const ReqDev* __restr
 #pragma unroll
       for (int i = 0; i < 4; ++i)
         if (bk[i] >= 0) H.order[base + bstart[bk[i]] + slot[i]] = (int32_t)(base + tid * 4 + i);
-      // chunk descriptors for non-empty buckets
-      uint64_t m = 0;
-      if (tid < kNumBuckets) m = ballot(hv > 0);
+      // chunk descriptors: each non-empty bucket in pieces of at most cmax
+      // entries (more, smaller work items when the hop is small); a piece's
+      // sums are its share of the bucket's, its maxima the bucket's
+      const uint32_t cmax = chunk_max_for(n, H.expand_waves);
+      uint32_t nck = tid < kNumBuckets ? (hv + cmax - 1) / cmax : 0u;
+      const uint32_t cincl = wave_incl_scan_u32(nck);
       __syncthreads();
-      if (tid < kNumBuckets && lane == 0) nz_cnt[wid] = (uint32_t)__popcll(m);
+      if (tid < kNumBuckets && lane == kWave - 1) nz_cnt[wid] = cincl;
       __syncthreads();
       if (tid == 0) chunk_base = atomicAdd(H.n_chunks, nz_cnt[0] + nz_cnt[1]);
       __syncthreads();
       if (tid < kNumBuckets && hv > 0) {
-        const uint32_t rank = (wid == 1 ? nz_cnt[0] : 0u) + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        const uint32_t ci = chunk_base + rank;
-        if (ci < H.chunk_cap) {
-          Chunk c;
-          c.start = (uint32_t)(base + bstart[tid]);
-          c.len = (uint16_t)hv;
-          c.bucket = (uint8_t)tid;
-          c.hop = (uint8_t)H.hop;
-          c.sum_f = fsum[tid];
-          c.max_d = dmax[tid];
-          c.sum_d = (float)dsum[tid];
-          c.key = (uint32_t)(blk * kNumBuckets + tid);
-          c.max_f = (uint8_t)fmax[tid];
-          c.pad = 0;
-          c.sum_l = (uint16_t)min(lsum[tid], 65535u);
-          H.chunks[ci] = c;
-        } else {
-          atomicOr(H.flags, 1u);
+        const uint32_t r0 = (wid == 1 ? nz_cnt[0] : 0u) + cincl - nck;
+        for (uint32_t pc = 0; pc < nck; ++pc) {
+          const uint32_t ci = chunk_base + r0 + pc;
+          const uint32_t len = min(cmax, hv - pc * cmax);
+          const float share = (float)len / (float)hv;
+          if (ci < H.chunk_cap) {
+            Chunk c;
+            c.start = (uint32_t)(base + bstart[tid] + pc * cmax);
+            c.len = (uint16_t)len;
+            c.bucket = (uint8_t)tid;
+            c.hop = (uint8_t)H.hop;
+            c.sum_f = (uint32_t)((float)fsum[tid] * share + 0.5f);
+            c.max_d = dmax[tid];
+            c.sum_d = (float)dsum[tid] * share;
+            c.key = (uint32_t)((blk * kNumBuckets + tid) * (kBlockEntries / kChunkMin) + pc);
+            c.max_f = (uint8_t)fmax[tid];
+            c.pad = 0;
+            c.sum_l = (uint16_t)min((uint32_t)((float)lsum[tid] * share + 0.5f), 65535u);
+            H.chunks[ci] = c;
+          } else {
+            atomicOr(H.flags, 1u);
+          }
         }
       }
       __syncthreads();
@@ -532,12 +539,14 @@ __global__ __launch_bounds__(256) void k_expand_wave(GraphDev G, csaw_graph_view
 }
 
 // Workgroup granularity: 256 lanes build the integer CDF (up to kBlockIntCap
-// entries in LDS), wave 0 resolves the slots.
-template <int KIND, bool BITMAP>
+// entries in LDS), wave 0 resolves the slots.  G32: the group-only build (see
+// k_expand_wave), for views whose CDF keys are all 32-bit.
+template <int KIND, bool BITMAP, bool G32>
 __global__ __launch_bounds__(256) void k_expand_block(GraphDev G, csaw_graph_view hv, const ReqDev* __restrict__ R,
                                                       const int32_t* __restrict__ gs2req, HopDev H, ExpandArgs A) {
-  __shared__ uint64_t cdf[KIND == kInt ? kBlockIntCap : 1];
-  __shared__ uint32_t bms[BITMAP ? kBlockBitmapCap / 32 : 1];
+  // (G32: only the four waves' small-CDF slots, 4 x kGroupLdsWords words)
+  __shared__ uint64_t cdf[KIND == kInt ? (G32 ? 2 * kGroupLdsWords : kBlockIntCap) : 1];
+  __shared__ uint32_t bms[BITMAP && !G32 ? kBlockBitmapCap / 32 : 1];
   __shared__ uint64_t wsum[4];
   __shared__ uint64_t carry;
   __shared__ uint32_t slot;
@@ -554,15 +563,19 @@ __global__ __launch_bounds__(256) void k_expand_block(GraphDev G, csaw_graph_vie
     if (!cc.valid) break;
     const uint64_t t0 = wall_clock64();
     const Chunk c = H.chunks[cc.ci];
-    if (groups_ok) {
+    if (G32 || groups_ok) {
       const int gs = group_size_for(c.max_f);
-      for (int k0 = wid * kWave; k0 < c.len; k0 += 4 * kWave)
-        group_expand_batch<KIND>(G, hv, R, gs2req, H, H.order, (int64_t)c.start + k0, min(kWave, (int)c.len - k0), gs,
-                                 gl);
+      for (int k0 = wid * kWave; k0 < c.len; k0 += 4 * kWave) {
+        const int64_t first = (int64_t)c.start + k0;
+        const int cnt = min(kWave, (int)c.len - k0);
+        if constexpr (G32) group_expand_batch<KIND, kKeys32>(G, hv, R, gs2req, H, H.order, first, cnt, gs, gl);
+        else group_expand_batch<KIND>(G, hv, R, gs2req, H, H.order, first, cnt, gs, gl);
+      }
       __syncthreads();
       log_chunk(A, H, cc.ci, var, t0);
       continue;
     }
+    // (G32 builds never get here: the workgroup-wide build below is dead code for them)
     for (int k = 0; k < c.len; ++k) {
       const Entry en = load_entry(G, R, gs2req, H, H.order[c.start + k]);
       if (en.d == 0 || en.F == 0) continue;
@@ -697,27 +710,29 @@ int launch_expand_chunks(const LaunchCfg& L, const GraphDev& G, const csaw_graph
       CSAW_TRY(go(kKernThread, k_expand_thread<kUniform>, L.thread_per_cu));
       CSAW_TRY(go(kKernWave, k_expand_wave<kUniform, true, true>, L.expand_per_cu));
       CSAW_TRY(go(kKernWave, k_expand_wave<kUniform, false, true>, L.expand_per_cu));
-      CSAW_TRY(go(kKernBlock, k_expand_block<kUniform, true>, L.block_per_cu));
-      CSAW_TRY(go(kKernBlock, k_expand_block<kUniform, false>, L.block_per_cu));
+      CSAW_TRY(go(kKernBlock, k_expand_block<kUniform, true, true>, L.block_per_cu));
+      CSAW_TRY(go(kKernBlock, k_expand_block<kUniform, false, true>, L.block_per_cu));
     } else if (kind == kInt) {
-      // group-only build when every integer cache of the view is uint32
+      // group-only builds when every integer cache of the view is uint32
       const bool g32 = A.int_cache && !G.int_cdf[0] && !G.int_cdf[1];
       CSAW_TRY(go(kKernThread, k_expand_thread<kInt>, L.thread_per_cu));
       if (g32) {
         CSAW_TRY(go(kKernWave, k_expand_wave<kInt, true, true>, L.expand_per_cu));
         CSAW_TRY(go(kKernWave, k_expand_wave<kInt, false, true>, L.expand_per_cu));
+        CSAW_TRY(go(kKernBlock, k_expand_block<kInt, true, true>, L.block_per_cu));
+        CSAW_TRY(go(kKernBlock, k_expand_block<kInt, false, true>, L.block_per_cu));
       } else {
         CSAW_TRY(go(kKernWave, k_expand_wave<kInt, true, false>, L.expand_per_cu));
         CSAW_TRY(go(kKernWave, k_expand_wave<kInt, false, false>, L.expand_per_cu));
+        CSAW_TRY(go(kKernBlock, k_expand_block<kInt, true, false>, L.block_per_cu));
+        CSAW_TRY(go(kKernBlock, k_expand_block<kInt, false, false>, L.block_per_cu));
       }
-      CSAW_TRY(go(kKernBlock, k_expand_block<kInt, true>, L.block_per_cu));
-      CSAW_TRY(go(kKernBlock, k_expand_block<kInt, false>, L.block_per_cu));
     } else {
       CSAW_TRY(go(kKernThread, k_expand_thread<kF32>, L.thread_per_cu));
       CSAW_TRY(go(kKernWave, k_expand_wave<kF32, true, true>, L.expand_per_cu));
       CSAW_TRY(go(kKernWave, k_expand_wave<kF32, false, true>, L.expand_per_cu));
-      CSAW_TRY(go(kKernBlock, k_expand_block<kF32, true>, L.block_per_cu));
-      CSAW_TRY(go(kKernBlock, k_expand_block<kF32, false>, L.block_per_cu));
+      CSAW_TRY(go(kKernBlock, k_expand_block<kF32, true, true>, L.block_per_cu));
+      CSAW_TRY(go(kKernBlock, k_expand_block<kF32, false, true>, L.block_per_cu));
     }
   }
   return CSAW_OK;

@@ -707,21 +707,26 @@ __device__ void expand_entry_thread(const GraphDev& G, const csaw_graph_view& hv
       }
       return lo;
     }
-    if (mode == kPickF32) return (int32_t)upper_bound_dev<float>(Cf, d, __fmul_rn(philox_u24(o), Tf));
-    if (mode == kPickF32Idx)
+    // (each kind compiles only its own modes: the code stays small)
+    if constexpr (KIND == kF32) {
+      if (mode == kPickF32) return (int32_t)upper_bound_dev<float>(Cf, d, __fmul_rn(philox_u24(o), Tf));
       return cdf_index_search<float>(G.cdf, Kf, G.E, en.o, d, __fmul_rn(philox_u24(o), Tf));
-    const uint64_t x = mulhi64(t, T);
-    if (mode == kPickInt) return (int32_t)upper_bound_dev<uint64_t>(Cg, d, x);
-    if (mode == kPickIntIdx) return cdf_index_search<uint64_t>(G.int_cdf[ck], Kg, G.E, en.o, d, x);
-    if (mode == kPickInt32Idx)
-      return cdf_index_search<uint32_t>(G.int_cdf32[ck], G.int_idx32[ck], G.E, en.o, d, (uint32_t)x);
-    uint64_t acc = 0;
-    int32_t i = 0;
-    for (; i < d - 1; ++i) {
-      acc += lds[i * stride];
-      if (acc > x) break;
     }
-    return i;
+    if constexpr (KIND == kInt) {
+      const uint64_t x = mulhi64(t, T);
+      if (mode == kPickInt) return (int32_t)upper_bound_dev<uint64_t>(Cg, d, x);
+      if (mode == kPickIntIdx) return cdf_index_search<uint64_t>(G.int_cdf[ck], Kg, G.E, en.o, d, x);
+      if (mode == kPickInt32Idx)
+        return cdf_index_search<uint32_t>(G.int_cdf32[ck], G.int_idx32[ck], G.E, en.o, d, (uint32_t)x);
+      uint64_t acc = 0;
+      int32_t i = 0;
+      for (; i < d - 1; ++i) {
+        acc += lds[i * stride];
+        if (acc > x) break;
+      }
+      return i;
+    }
+    return 0;
   };
   // Four slots at a time: their attempt-0 searches advance in lockstep, so
   // each lane keeps up to four independent loads in flight.
@@ -751,7 +756,7 @@ __device__ void expand_entry_thread(const GraphDev& G, const csaw_graph_view& hv
         for (int q = 0; q < G4; ++q) xf[q] = q < nq ? __fmul_rn(philox_u24(draw(en, j0 + q, 0)), Tf) : 0.f;
         cdf_index_search_n<float, G4>(G.cdf, Kf, G.E, en.o, d, xf, nq, out);
       }
-    } else if (mode == kPickInt || mode == kPickF32) {
+    } else if (KIND != kUniform && (mode == kPickInt || mode == kPickF32)) {
       int32_t lo[G4], hi[G4];
       uint64_t xi[G4];
       float xf[G4];
@@ -793,7 +798,8 @@ __device__ void expand_entry_thread(const GraphDev& G, const csaw_graph_view& hv
 #pragma unroll
       for (int q = 0; q < G4; ++q) out[q] = lo[q];
     } else {
-#pragma unroll
+      // one copy of pick's code (an unrolled loop would inline it four times)
+#pragma unroll 1
       for (int q = 0; q < G4; ++q) out[q] = q < nq ? pick(draw(en, j0 + q, 0)) : 0;
     }
   };
@@ -810,15 +816,6 @@ __device__ void expand_entry_thread(const GraphDev& G, const csaw_graph_view& hv
         H.smp[p] = en.g;
       }
   };
-  if (!en.dedup) {
-    for (int j0 = 0; j0 < en.F; j0 += G4) {
-      const int nq = min(G4, en.F - j0);
-      int32_t idx[G4];
-      pick4(j0, nq, idx);
-      emit4(j0, nq, idx);
-    }
-    return;
-  }
   const bool use_mask = d <= 64;
   uint64_t mask = 0;
   int n_taken = 0;
@@ -828,13 +825,16 @@ __device__ void expand_entry_thread(const GraphDev& G, const csaw_graph_view& hv
       if ((int32_t)lds[m * stride] == i) return true;
     return false;
   };
+  // one loop for both modes, so pick4 and emit4 are inlined once
   for (int j0 = 0; j0 < en.F; j0 += G4) {
     const int nq = min(G4, en.F - j0);
     int32_t idx[G4];
     pick4(j0, nq, idx);
-#pragma unroll
+    // not unrolled: the reselect loop (with pick inlined) stays one copy, so the
+    // kernel's hot code fits the instruction cache
+#pragma unroll 1
     for (int q = 0; q < G4; ++q) {
-      if (q >= nq) break;
+      if (q >= nq || !en.dedup) break;
       int32_t i = idx[q];
       if (taken(i)) {
         bool found = false;
