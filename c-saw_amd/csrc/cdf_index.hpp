@@ -112,6 +112,18 @@ __host__ __device__ __forceinline__ int32_t cdf_index_search(const T* __restrict
   return (int32_t)(b - o);
 }
 
+// The 16-key block a search over C[o .. o+d) reads first (the same for every
+// target), so a caller can load it ahead of the draws.
+template <typename T>
+__host__ __device__ __forceinline__ const T* cdf_index_first_block(const T* C, const T* K, int64_t E, int64_t o,
+                                                                   int32_t d) {
+  const int L = cdf_index_reads(o, d);
+  if (L <= 1) return C + 16 * (o >> 4);
+  int64_t at = 0;
+  for (int m = 1; m <= L - 2; ++m) at += cdf_index_level_size(E, m);
+  return K + at + 16 * (o >> (4 * L));
+}
+
 // NQ searches over the same range in lockstep (one round of loads per level
 // for all of them); queries q >= nq are skipped.
 template <typename T, int NQ>

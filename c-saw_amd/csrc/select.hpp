@@ -474,6 +474,7 @@ __device__ __forceinline__ void gwrite_edge(const GraphDev& G, const csaw_graph_
 // ci equal to one of the group's resolved slots 0 .. jj-1 (res of lane gbase+m)
 __device__ __forceinline__ bool group_taken(int32_t ci, int32_t res, int jj, int gbase) {
   bool hit = false;
+#pragma unroll 8
   for (int m = 0; m < jj; ++m) hit |= (ci == shfl_abs(res, gbase + m));
   return hit;
 }
@@ -520,7 +521,9 @@ __device__ int32_t group_select(const GraphDev& G, const GEntry& en, int gs, int
   int32_t cand = pick(gl, 0, act);
   if (!act) cand = -1;
   if (!(en.flags & 1u)) return cand;
+  // duplicate check: shuffles issued eight at a time (one wait per eight)
   bool dup = false;
+#pragma unroll 8
   for (int i = 0; i + 1 < F; ++i) {
     const int32_t ci = shfl_abs(cand, gbase + i);
     dup |= (gl > i) & act & (cand == ci);
@@ -529,15 +532,32 @@ __device__ int32_t group_select(const GraphDev& G, const GEntry& en, int gs, int
   if (dm == 0ull) return cand;
   const int j0 = __builtin_ctzll(dm) - gbase;
   int32_t res = gl < j0 ? cand : -1;
+  // Taken set: for d <= 128 a 128-bit mask held identically by the group's
+  // lanes (O(1) tests, no shuffles; where collisions are frequent), else the
+  // resolved lanes' registers.
+  const bool use_mask = d <= 128;
+  uint64_t mlo = 0, mhi = 0;
+  if (use_mask) {
+    uint64_t blo = (gl < j0 && cand < 64) ? (1ull << cand) : 0ull;
+    uint64_t bhi = (gl < j0 && cand >= 64) ? (1ull << (cand - 64)) : 0ull;
+    for (int off = 1; off < gs; off <<= 1) {  // OR over the group (lane ^ off stays in it)
+      blo |= shfl_abs_u64(blo, lane_id() ^ off);
+      bhi |= shfl_abs_u64(bhi, lane_id() ^ off);
+    }
+    mlo = blo;
+    mhi = bhi;
+  }
+  auto in_mask = [&](int32_t ci) -> bool { return ((ci < 64 ? mlo >> ci : mhi >> (ci - 64)) & 1ull) != 0ull; };
   for (int jj = j0; jj < F; ++jj) {
     int32_t c = shfl_abs(cand, gbase + jj);
-    if ((ballot(res == c) & gmask) != 0ull) {
+    const bool c_taken = use_mask ? in_mask(c) : ((ballot(res == c) & gmask) != 0ull);
+    if (c_taken) {
       int32_t chosen = -1, last = c;
       for (int a0 = 1; a0 < kReselect; a0 += gs) {
         const int a = a0 + gl;
         int32_t ci = pick(jj, a, a < kReselect);
         if (a >= kReselect) ci = 0;
-        const bool tk = group_taken(ci, res, jj, gbase);
+        const bool tk = use_mask ? in_mask(ci) : group_taken(ci, res, jj, gbase);
         const uint64_t m = ballot((a < kReselect) && !tk) & gmask;
         if (m) {
           chosen = shfl_abs(ci, __builtin_ctzll(m));
@@ -549,7 +569,7 @@ __device__ int32_t group_select(const GraphDev& G, const GEntry& en, int gs, int
       if (chosen < 0) {
         for (int q = 0; q < d; q += gs) {
           const int32_t i = (int32_t)(((int64_t)last + 1 + q + gl) % d);
-          const bool tk = group_taken(i, res, jj, gbase);
+          const bool tk = use_mask ? in_mask(i) : group_taken(i, res, jj, gbase);
           const uint64_t m = ballot((q + gl < d) && !tk) & gmask;
           if (m) {
             chosen = shfl_abs(i, __builtin_ctzll(m));
@@ -558,6 +578,10 @@ __device__ int32_t group_select(const GraphDev& G, const GEntry& en, int gs, int
         }
       }
       c = chosen;
+    }
+    if (use_mask) {
+      if (c < 64) mlo |= 1ull << c;
+      else mhi |= 1ull << (c - 64);
     }
     if (gl == jj) res = c;
   }
@@ -588,8 +612,24 @@ __device__ void group_expand_batch(const GraphDev& G, const csaw_graph_view& hv,
   GEntry mine;
   mine.d = 0;
   mine.F = 0;
-  if (lane < count)
+  uint32_t warm = 0;
+  if (lane < count) {
     mine = make_gentry<KIND>(G, load_entry(G, R, gs2req, H, order ? (int64_t)order[first + lane] : first + lane));
+    // Every entry's first search block (or, for small degrees, its CDF) is
+    // loaded now, for all 64 entries at once, so the rounds below find it in
+    // cache instead of each waiting for memory.
+    const uint32_t mode = (mine.flags >> 4) & 15u;
+    if (KIND == kF32 && mode == kPickF32Idx)
+      warm = *reinterpret_cast<const uint32_t*>(
+          mine.d <= kGroupSmallCap ? G.cdf + mine.o : cdf_index_first_block<float>(G.cdf, G.cdf_idx, G.E, mine.o, mine.d));
+    if (KIND == kInt && mode == kPickInt32Idx) {
+      const bool user = (mine.flags >> 1) & 1u;
+      const uint32_t* C = user ? G.int_cdf32[1] : G.int_cdf32[0];
+      warm = mine.d <= kGroupSmallCap
+                 ? C[mine.o]
+                 : *cdf_index_first_block<uint32_t>(C, user ? G.int_idx32[1] : G.int_idx32[0], G.E, mine.o, mine.d);
+    }
+  }
   // Software pipeline: a round's neighbour load is consumed (written out) only
   // after the next round's selection, so its latency overlaps that round's
   // searches instead of adding to them.
@@ -626,6 +666,9 @@ __device__ void group_expand_batch(const GraphDev& G, const csaw_graph_view& hv,
     H.dst[pp] = pnb;
     H.smp[pp] = pg;
   }
+  // consume the warm-up loads where their wait costs nothing (never true:
+  // count >= 0; the compiler cannot drop the loads)
+  if (count < 0 && warm == 0x5eedu) atomicOr(H.flags, 0u);
 }
 
 // ---------------------------------------------------------------------------
