@@ -250,7 +250,7 @@ def main():
     ap.add_argument("--workload", choices=["c4", "c1", "c2", "c3", "uniform", "c5"], default="c4",
                     help="c4 (default) is the headline; the others measure the remaining configs")
     ap.add_argument("--no-graph", action="store_true", help="--only batched: launch eagerly (profilers)")
-    ap.add_argument("--only", choices=["all", "batched", "plain", "copy"], default="all",
+    ap.add_argument("--only", choices=["all", "batched", "plain", "copy", "gather"], default="all",
                     help="profiling modes: run one executor only (or a 256 MiB copy for counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -266,6 +266,24 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4,
                           "launches": a.steps}))
+        return
+    if a.only == "gather":
+        # random-read ceiling: 64M independent random 4-byte reads from a
+        # 512 MiB array (the size of the RMAT-22 neighbour array)
+        import torch
+        x = torch.ones(128 << 20, dtype=torch.int32, device="cuda")
+        idx = torch.randint(0, x.numel(), (64 << 20,), device="cuda", dtype=torch.int32)
+        y = x[idx]
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(a.steps):
+            y = x[idx]
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / a.steps
+        print(json.dumps({"random_reads": idx.numel(), "ms": ms, "reads_per_s": idx.numel() / (ms * 1e-3),
+                          "line_bytes_per_s_at_64B": idx.numel() * 64 / (ms * 1e-3)}))
         return
     import __graft_entry__ as ge
     C = ge.load_csaw()
