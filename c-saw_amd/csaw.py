@@ -432,7 +432,12 @@ class Mlp:
 # --------------------------------------------------------------- streaming
 class StreamStats(ctypes.Structure):
     _fields_ = [("edges", i64), ("device_ms", ctypes.c_double), ("h2d_bytes", ctypes.c_double),
-                ("partitions", i32), ("peak_device_bytes", i64), ("slice_bytes", i64)]
+                ("partitions", i32), ("peak_device_bytes", i64), ("slice_bytes", i64),
+                ("expand_ms", ctypes.c_double), ("chunks", i64), ("variant_chunks", i64 * NUM_VARIANTS)]
+
+
+class StreamOpts(ctypes.Structure):
+    _fields_ = [("mode", i32), ("sched", i32), ("variant", i32), ("explore_key", u64), ("mlp", P), ("tuning", P)]
 
 
 _sig("csaw_host_graph_create_rmat", [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -441,6 +446,7 @@ _sig("csaw_host_graph_from_csr", [P, P, i64, i64, P])
 _sig("csaw_host_graph_info", [P, P, P, P, P, P])
 _sig("csaw_host_graph_destroy", [P])
 _sig("csaw_stream_sample", [P, P, i64, i32, P, P])
+_sig("csaw_stream_sample_opts", [P, P, i64, i32, P, P, P])
 _sig("csaw_edges_from_host", [i32, i32, i64, P, P, P, P, P])
 _sig("csaw_mdrw", [P, u64, u32, i32, i32, i32, P, P, P, P, P])
 
@@ -491,13 +497,19 @@ class HostGraph:
 
 
 def stream_sample(hg, seeds, fanouts, bias=BIAS_DEGREE, dedup=1, key=42, request=0, budget_bytes=4 << 30,
-                  partitions=16):
+                  partitions=16, mode=MODE_BATCHED, sched=SCHED_HEURISTIC, variant=0, mlp=None, explore_key=0,
+                  tuning=None):
+    """Streams one request; mode / sched / variant / mlp choose how each
+    partition's entries are expanded (the edge list is the same for all)."""
     reqs, keep = make_requests([{"seeds": seeds, "fanouts": fanouts, "bias": bias, "dedup": dedup, "key": key,
                                  "request": request}])
     e = P()
     st = StreamStats()
-    _check(_L.csaw_stream_sample(hg.h, reqs, budget_bytes, partitions, ctypes.byref(e), ctypes.byref(st)),
-           "stream_sample")
+    o = StreamOpts(mode=mode, sched=sched, variant=variant, explore_key=explore_key,
+                   mlp=mlp.h if mlp is not None else None,
+                   tuning=ctypes.cast(ctypes.byref(tuning), P) if tuning is not None else None)
+    _check(_L.csaw_stream_sample_opts(hg.h, reqs, budget_bytes, partitions, ctypes.byref(o), ctypes.byref(e),
+                                      ctypes.byref(st)), "stream_sample")
     try:
         n, d, s = i64(), i32(), i32()
         _check(_L.csaw_edges_info(e, ctypes.byref(n), ctypes.byref(d), ctypes.byref(s)), "edges_info")

@@ -22,9 +22,12 @@
 namespace csaw {
 
 // ------------------------------------------------------------------ count
-template <bool BIN>
+// LIST: bins only (no counts) the n_list entries list[0 .. n_list) of one
+// partition; order then holds entry ids by list position.
+template <bool BIN, bool LIST = false>
 __global__ __launch_bounds__(256) void k_count(GraphDev G, const ReqDev* __restrict__ R, const int32_t* __restrict__ gs2req,
-                                               HopDev H) {
+                                               HopDev H, const int32_t* __restrict__ list = nullptr,
+                                               int64_t n_list = 0) {
   __shared__ uint32_t wsum[4];
   __shared__ uint32_t hist[kNumBuckets];
   __shared__ uint32_t bstart[kNumBuckets];
@@ -35,7 +38,7 @@ __global__ __launch_bounds__(256) void k_count(GraphDev G, const ReqDev* __restr
   __shared__ uint32_t lsum[kNumBuckets];
   __shared__ uint32_t nz_cnt[2];
   __shared__ uint32_t chunk_base;
-  const int64_t n = *H.n_ptr;
+  const int64_t n = LIST ? n_list : *H.n_ptr;
   const int64_t nblk = (n + kBlockEntries - 1) >> kBlockShift;
   const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
   for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
@@ -49,14 +52,18 @@ __global__ __launch_bounds__(256) void k_count(GraphDev G, const ReqDev* __restr
     uint32_t k[4];
     int bk[4];
     uint32_t slot[4];
+    int32_t eid[4];
     uint32_t tsum = 0;
     unsigned long long tbytes = 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int64_t e = base + tid * 4 + i;
+      const int64_t pos = base + tid * 4 + i;
       k[i] = 0;
       bk[i] = -1;
-      if (e < n) {
+      eid[i] = 0;
+      if (pos < n) {
+        const int64_t e = LIST ? (int64_t)list[pos] : pos;
+        eid[i] = (int32_t)e;
         const int32_t v = H.fv[e];
         const ReqDev& r = R[gs2req[H.fs[e]]];
         const int32_t F = r.fanout[H.hop];
@@ -86,19 +93,23 @@ __global__ __launch_bounds__(256) void k_count(GraphDev G, const ReqDev* __restr
       for (int off = 32; off >= 1; off >>= 1) tbytes += __shfl_xor(tbytes, off, 64);
       if (lane == 0 && tbytes) atomicAdd(&H.bytes[kKernPlain], tbytes);
     }
-    // block exclusive scan of per-thread sums
-    const uint32_t incl = wave_incl_scan_u32(tsum);
-    if (lane == kWave - 1) wsum[wid] = incl;
-    __syncthreads();
-    uint32_t pre = incl - tsum;
-    for (int w = 0; w < wid; ++w) pre += wsum[w];
+    if (!LIST) {
+      // block exclusive scan of per-thread sums
+      const uint32_t incl = wave_incl_scan_u32(tsum);
+      if (lane == kWave - 1) wsum[wid] = incl;
+      __syncthreads();
+      uint32_t pre = incl - tsum;
+      for (int w = 0; w < wid; ++w) pre += wsum[w];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int64_t e = base + tid * 4 + i;
-      if (e < n) H.local_excl[e] = pre;
-      pre += k[i];
+      for (int i = 0; i < 4; ++i) {
+        const int64_t e = base + tid * 4 + i;
+        if (e < n) H.local_excl[e] = pre;
+        pre += k[i];
+      }
+      if (tid == 255) H.blk_excl[blk] = pre;  // block total, scanned later
+    } else {
+      __syncthreads();  // the histogram is complete
     }
-    if (tid == 255) H.blk_excl[blk] = pre;  // block total, scanned later
     if (BIN) {
       // exclusive scan of the bucket histogram (threads 0..127 own one bucket)
       uint32_t hv = 0, hs = 0;
@@ -112,7 +123,7 @@ __global__ __launch_bounds__(256) void k_count(GraphDev G, const ReqDev* __restr
       __syncthreads();
 #pragma unroll
       for (int i = 0; i < 4; ++i)
-        if (bk[i] >= 0) H.order[base + bstart[bk[i]] + slot[i]] = (int32_t)(base + tid * 4 + i);
+        if (bk[i] >= 0) H.order[base + bstart[bk[i]] + slot[i]] = eid[i];
       // chunk descriptors: each non-empty bucket in pieces of at most cmax
       // entries (more, smaller work items when the hop is small); a piece's
       // sums are its share of the bucket's, its maxima the bucket's
@@ -646,6 +657,16 @@ int launch_count(const LaunchCfg& L, const GraphDev& G, const ReqDev* R, const i
   int grid = (int)std::max<int64_t>(1, std::min<int64_t>(nb, grid_of(L, L.count_per_cu)));
   if (bin) k_count<true><<<grid, 256, 0, L.stream>>>(G, R, gs2req, H);
   else k_count<false><<<grid, 256, 0, L.stream>>>(G, R, gs2req, H);
+  CSAW_LAUNCH_CHECK();
+  return CSAW_OK;
+}
+
+int launch_bin_list(const LaunchCfg& L, const GraphDev& G, const ReqDev* R, const int32_t* gs2req, const HopDev& H,
+                    const int32_t* list, int64_t n) {
+  if (n <= 0) return CSAW_OK;
+  int64_t nb = (n + kBlockEntries - 1) / kBlockEntries;
+  int grid = (int)std::max<int64_t>(1, std::min<int64_t>(nb, grid_of(L, L.count_per_cu)));
+  k_count<true, true><<<grid, 256, 0, L.stream>>>(G, R, gs2req, H, list, n);
   CSAW_LAUNCH_CHECK();
   return CSAW_OK;
 }

@@ -80,6 +80,9 @@ __device__ int mlp_choose(const ChooseArgs& A, const float* wl, const Chunk& c, 
 
 int launch_count(const LaunchCfg& L, const GraphDev& G, const ReqDev* R, const int32_t* gs2req, const HopDev& H,
                  bool bin, int64_t max_entries);
+// Bins the n entries list[0 .. n) into chunks (H.order by list position).
+int launch_bin_list(const LaunchCfg& L, const GraphDev& G, const ReqDev* R, const int32_t* gs2req, const HopDev& H,
+                    const int32_t* list, int64_t n);
 int launch_scans(const LaunchCfg& L, const HopDev& H, int64_t* total, int64_t max_entries);
 int launch_fill(const LaunchCfg& L, uint32_t* p, uint32_t v, int64_t n);
 int launch_digest(hipStream_t st, int num_cu, const int32_t* src, const int32_t* dst, const int32_t* smp, int64_t lo,

@@ -13,6 +13,11 @@
 //    per (hop, partition) into a pool, split across workgroups;
 //  - output positions come from the same global prefix sums as the
 //    in-memory run, so the edge list is bit-identical to it.
+// In batched mode each partition's entries are binned into chunks by degree
+// class and every chunk runs the kernel variant the schedule picks (the
+// fixed heuristic, a forced variant, a random one or the learned cost
+// model), exactly as in the in-memory batched run; in plain mode one wave
+// expands each listed entry.
 // Every device allocation is charged against budget_bytes.
 #include <algorithm>
 #include <cstring>
@@ -20,6 +25,7 @@
 #include <vector>
 
 #include "graph.hpp"
+#include "plan.hpp"
 #include "sample_kernels.hpp"
 
 struct csaw_host_graph {
@@ -181,10 +187,24 @@ extern "C" int csaw_host_graph_destroy(csaw_host_graph* g) {
   return CSAW_OK;
 }
 
-extern "C" int csaw_stream_sample(csaw_host_graph* hg, const csaw_request* req, int64_t budget_bytes,
-                                  int32_t n_partitions, csaw_edges** out, csaw_stream_stats* stats) {
+extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* req, int64_t budget_bytes,
+                                       int32_t n_partitions, const csaw_stream_opts* opts, csaw_edges** out,
+                                       csaw_stream_stats* stats) {
   if (!hg || !req || !out || n_partitions < 1 || n_partitions > 255 || budget_bytes <= 0) return CSAW_ERR_INVALID_ARG;
   *out = nullptr;
+  csaw_stream_opts o;
+  memset(&o, 0, sizeof(o));
+  o.mode = CSAW_MODE_BATCHED;
+  o.sched = CSAW_SCHED_HEURISTIC;
+  if (opts) o = *opts;
+  if (o.mode != CSAW_MODE_PLAIN && o.mode != CSAW_MODE_BATCHED) return CSAW_ERR_INVALID_ARG;
+  if (o.sched < CSAW_SCHED_HEURISTIC || o.sched > CSAW_SCHED_RANDOM) return CSAW_ERR_INVALID_ARG;
+  if (o.sched == CSAW_SCHED_FORCED && (o.variant < 0 || o.variant >= CSAW_NUM_VARIANTS)) return CSAW_ERR_INVALID_ARG;
+  if (o.mode == CSAW_MODE_BATCHED && o.sched == CSAW_SCHED_MLP && !o.mlp) {
+    set_last_error("CSAW_SCHED_MLP needs a trained csaw_mlp");
+    return CSAW_ERR_INVALID_ARG;
+  }
+  const bool batched = o.mode == CSAW_MODE_BATCHED;
   const csaw_request& r = *req;
   if (r.samples < 0 || r.depth < 0 || r.depth > CSAW_MAX_HOPS || (r.samples > 0 && !r.seeds)) return CSAW_ERR_INVALID_ARG;
   if (r.bias == CSAW_BIAS_WEIGHT) {
@@ -232,9 +252,12 @@ extern "C" int csaw_stream_sample(csaw_host_graph* hg, const csaw_request* req, 
   Budget B(budget_bytes);
   hipStream_t cs = nullptr, xs = nullptr;
   hipEvent_t ready[2] = {nullptr, nullptr}, freed[2] = {nullptr, nullptr}, t0 = nullptr, t1 = nullptr;
+  std::vector<hipEvent_t> evs;  // per processed partition: bin begin/end, expand begin/end
+  uint32_t* h_qlen = nullptr;   // pinned: queue lengths per (hop, partition)
   std::vector<int32_t> h_src, h_dst, h_smp;
   std::vector<int64_t> h_ho;
-  double h2d_bytes = 0, h2d_ms = 0;
+  double h2d_bytes = 0, h2d_ms = 0, expand_ms = 0;
+  int64_t n_chunks_all = 0, var_chunks[CSAW_NUM_VARIANTS] = {};
   auto body = [&]() -> int {
     CSAW_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     CSAW_HIP(hipStreamCreateWithFlags(&xs, hipStreamNonBlocking));
@@ -320,6 +343,32 @@ extern "C" int csaw_stream_sample(csaw_host_graph* hg, const csaw_request* req, 
     d_cursor = d_cnt + 2 * P;
     CSAW_TRY(B.alloc(&d_vb, P + 1));
     CSAW_HIP(hipMemcpyAsync(d_vb, vb.data(), sizeof(int32_t) * (P + 1), hipMemcpyHostToDevice, cs));
+    LaunchCfg L;
+    L.stream = cs;
+    L.num_cu = num_cu;
+    if (o.tuning) apply_tuning(L, o.tuning);
+    // batched mode: chunk descriptors of one partition, variant queues, counters
+    int32_t* d_order = nullptr;
+    Chunk* d_chunks = nullptr;
+    uint32_t *d_queues = nullptr, *d_ctr = nullptr;
+    uint16_t* d_mlp = nullptr;
+    uint32_t chunk_cap = 0;
+    const int32_t expand_waves = L.num_cu * L.expand_per_cu * 4;
+    if (batched) {
+      chunk_cap = (uint32_t)std::min<int64_t>(nblk * kChunksPerBlock + expand_waves, 0xFFFFFFF0LL);
+      CSAW_TRY(B.alloc(&d_order, mx));
+      CSAW_TRY(B.alloc(&d_chunks, chunk_cap));
+      CSAW_TRY(B.alloc(&d_queues, (int64_t)kNumQueues * chunk_cap));
+      CSAW_TRY(B.alloc(&d_ctr, kCtrCount));
+      CSAW_TRY(launch_fill(L, d_ctr, 0u, kCtrCount));
+      if (o.sched == CSAW_SCHED_MLP) {
+        CSAW_TRY(B.alloc(&d_mlp, kMlpParams));
+        CSAW_TRY(mlp_upload_bf16(o.mlp, d_mlp));
+      }
+      CSAW_HIP(hipHostMalloc((void**)&h_qlen, sizeof(uint32_t) * kNumQueues * std::max(r.depth, 1) * P,
+                             hipHostMallocDefault));
+      memset(h_qlen, 0, sizeof(uint32_t) * kNumQueues * std::max(r.depth, 1) * P);
+    }
 
     GraphDev G;
     memset(&G, 0, sizeof(G));
@@ -334,10 +383,14 @@ extern "C" int csaw_stream_sample(csaw_host_graph* hg, const csaw_request* req, 
     hv.weight_list = nullptr;
     hv.vert_count = V;
     hv.edge_count = E;
-    LaunchCfg L;
-    L.stream = cs;
-    L.num_cu = num_cu;
     std::vector<uint32_t> hubbits_h;
+    bool kinds_present[3] = {false, false, false};
+    kinds_present[kind] = true;
+    auto event = [&](hipEvent_t* e) -> int {
+      CSAW_HIP(hipEventCreate(e));
+      evs.push_back(*e);
+      return CSAW_OK;
+    };
     CSAW_HIP(hipEventRecord(t0, cs));
     for (int h = 0; h < r.depth; ++h) {
       HopDev H;
@@ -355,6 +408,14 @@ extern "C" int csaw_stream_sample(csaw_host_graph* hg, const csaw_request* req, 
       H.dst = d_dst[h];
       H.smp = d_smp[h];
       H.eo = d_eo[h];
+      if (batched) {
+        H.order = d_order;
+        H.chunks = d_chunks;
+        H.n_chunks = d_ctr + kCtrChunks;
+        H.chunk_cap = chunk_cap;
+        H.flags = d_ctr + kCtrFlags;
+        H.expand_waves = expand_waves;
+      }
       CSAW_TRY(launch_count(L, G, d_req, d_gs2req, H, false, nmax[h]));
       CSAW_TRY(launch_scans(L, H, d_tot + h, nmax[h]));
       // bin the frontier by partition
@@ -405,7 +466,37 @@ extern "C" int csaw_stream_sample(csaw_host_graph* hg, const csaw_request* req, 
       for (size_t t = 0; t < todo.size(); ++t) {
         const int p = todo[t], buf = (int)(t & 1);
         if (t + 1 < todo.size()) CSAW_TRY(issue_copy(todo[t + 1], (int)((t + 1) & 1)));
+        hipEvent_t eb0, eb1, ex0, ex1;
+        CSAW_TRY(event(&eb0));
+        CSAW_TRY(event(&eb1));
+        CSAW_TRY(event(&ex0));
+        CSAW_TRY(event(&ex1));
+        CSAW_HIP(hipEventRecord(eb0, cs));
+        if (batched) {
+          // chunks of this partition's entries and their variants (needs only
+          // degrees, so it runs while the slice is still in flight)
+          CSAW_TRY(launch_fill(L, d_ctr, 0u, kCtrCount - 1));  // keeps the flags word
+          CSAW_TRY(launch_bin_list(L, G, d_req, d_gs2req, H, d_list + poff[p], (int64_t)cnt[p]));
+          ChooseArgs A;
+          memset(&A, 0, sizeof(A));
+          A.sched = o.sched;
+          A.variant = o.variant;
+          A.explore_key = o.explore_key;
+          A.qlen = d_ctr + kCtrQlen;
+          A.queues = d_queues;
+          A.queue_cap = chunk_cap;
+          A.chosen = nullptr;
+          A.int_cache = 0;
+          A.mlp.w = d_mlp;
+          A.mlp_w = d_mlp;
+          const int64_t nblk_p = ((int64_t)cnt[p] + kBlockEntries - 1) / kBlockEntries;
+          CSAW_TRY(launch_choose(L, A, H, std::min<int64_t>(nblk_p * kChunksPerBlock + expand_waves, chunk_cap)));
+          CSAW_HIP(hipMemcpyAsync(h_qlen + ((size_t)h * P + p) * kNumQueues, d_ctr + kCtrQlen,
+                                  sizeof(uint32_t) * kNumQueues, hipMemcpyDeviceToHost, cs));
+        }
+        CSAW_HIP(hipEventRecord(eb1, cs));
         CSAW_HIP(hipStreamWaitEvent(cs, ready[buf], 0));
+        CSAW_HIP(hipEventRecord(ex0, cs));
         GraphDev Gp = G;
         Gp.adj = slice[buf] - hg->off[vb[p]];  // global offsets index the slice
         csaw_graph_view hvp = hv;
@@ -424,7 +515,19 @@ extern "C" int csaw_stream_sample(csaw_host_graph* hg, const csaw_request* req, 
         } else {
           Gp.n_hubs = 0;
         }
-        CSAW_TRY(launch_expand_list(L, kind, Gp, hvp, d_req, d_gs2req, H, d_list + poff[p], (int64_t)cnt[p]));
+        if (batched) {
+          ExpandArgs X;
+          X.queues = d_queues;
+          X.queue_cap = chunk_cap;
+          X.qlen = d_ctr + kCtrQlen;
+          X.qhead = d_ctr + kCtrQhead;
+          X.log = nullptr;
+          X.int_cache = 0;
+          CSAW_TRY(launch_expand_chunks(L, Gp, hvp, d_req, d_gs2req, H, X, kinds_present, nullptr));
+        } else {
+          CSAW_TRY(launch_expand_list(L, kind, Gp, hvp, d_req, d_gs2req, H, d_list + poff[p], (int64_t)cnt[p]));
+        }
+        CSAW_HIP(hipEventRecord(ex1, cs));
         CSAW_HIP(hipEventRecord(freed[buf], cs));
         if (int_bias && !hubs[p].empty()) CSAW_HIP(hipStreamSynchronize(cs));  // hub list buffers are reused
       }
@@ -434,6 +537,25 @@ extern "C" int csaw_stream_sample(csaw_host_graph* hg, const csaw_request* req, 
     float ms = 0.f;
     CSAW_HIP(hipEventElapsedTime(&ms, t0, t1));
     h2d_ms = ms;
+    for (size_t i = 0; i + 1 < evs.size(); i += 2) {
+      float m = 0.f;
+      CSAW_HIP(hipEventElapsedTime(&m, evs[i], evs[i + 1]));
+      expand_ms += m;
+    }
+    if (batched) {
+      uint32_t flags = 0;
+      CSAW_HIP(hipMemcpy(&flags, d_ctr + kCtrFlags, sizeof(uint32_t), hipMemcpyDeviceToHost));
+      if (flags & 1u) {
+        set_last_error("chunk descriptor capacity exceeded");
+        return CSAW_ERR_CAPACITY;
+      }
+      for (int64_t i = 0; i < (int64_t)std::max(r.depth, 1) * P; ++i)
+        for (int q = 0; q < kNumQueues; ++q) {
+          const uint32_t c = h_qlen[i * kNumQueues + q];
+          n_chunks_all += c;
+          if (q != kQueueCopy) var_chunks[q / 3] += c;
+        }
+    }
     // copy out
     std::vector<int64_t> lo(r.depth), hi(r.depth);
     int64_t n_all = 0;
@@ -468,10 +590,12 @@ extern "C" int csaw_stream_sample(csaw_host_graph* hg, const csaw_request* req, 
   }
   if (t0) (void)hipEventDestroy(t0);
   if (t1) (void)hipEventDestroy(t1);
+  for (hipEvent_t e : evs) (void)hipEventDestroy(e);
   if (cs) (void)hipStreamSynchronize(cs);
   if (xs) (void)hipStreamSynchronize(xs);
   if (cs) (void)hipStreamDestroy(cs);
   if (xs) (void)hipStreamDestroy(xs);
+  if (h_qlen) (void)hipHostFree(h_qlen);
   if (st != CSAW_OK) return st;
   if (stats) {
     memset(stats, 0, sizeof(*stats));
@@ -481,7 +605,15 @@ extern "C" int csaw_stream_sample(csaw_host_graph* hg, const csaw_request* req, 
     stats->partitions = P;
     stats->peak_device_bytes = B.peak;
     stats->slice_bytes = slice_max * (int64_t)sizeof(int32_t);
+    stats->expand_ms = expand_ms;
+    stats->chunks = n_chunks_all;
+    for (int v = 0; v < CSAW_NUM_VARIANTS; ++v) stats->variant_chunks[v] = var_chunks[v];
   }
   return csaw_edges_from_host(r.depth, S, (int64_t)h_src.size(), h_src.data(), h_dst.data(), h_smp.data(),
                               h_ho.data(), out);
+}
+
+extern "C" int csaw_stream_sample(csaw_host_graph* hg, const csaw_request* req, int64_t budget_bytes,
+                                  int32_t n_partitions, csaw_edges** out, csaw_stream_stats* stats) {
+  return csaw_stream_sample_opts(hg, req, budget_bytes, n_partitions, nullptr, out, stats);
 }

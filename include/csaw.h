@@ -275,12 +275,35 @@ typedef struct csaw_stream_stats {
   int32_t partitions;
   int64_t peak_device_bytes; /* budget charged at the peak                  */
   int64_t slice_bytes;       /* largest partition slice                     */
+  double expand_ms;          /* device time of the per-partition work:      */
+                             /*  binning, variant choice, hub CDFs and      */
+                             /*  expansion (copy waits excluded)            */
+  int64_t chunks;            /* chunks scheduled (batched mode)             */
+  int64_t variant_chunks[CSAW_NUM_VARIANTS]; /* chunks run per variant      */
 } csaw_stream_stats;
+
+/* How each partition's frontier entries are expanded. */
+typedef struct csaw_stream_opts {
+  int32_t mode;         /* CSAW_MODE_BATCHED: entries binned into chunks by  */
+                        /*  degree class and each chunk run by the variant  */
+                        /*  `sched` picks; CSAW_MODE_PLAIN: one wave per     */
+                        /*  listed entry                                     */
+  int32_t sched;        /* csaw_sched                                        */
+  int32_t variant;      /* forced variant                                    */
+  uint64_t explore_key; /* key of CSAW_SCHED_RANDOM                          */
+  const struct csaw_mlp *mlp;       /* cost model for CSAW_SCHED_MLP or NULL */
+  const struct csaw_tuning *tuning; /* launch shapes or NULL (defaults)      */
+} csaw_stream_opts;
 
 /* Samples one request (uniform, degree or user bias) streaming vertex-range
  * partitions through two device buffers under budget_bytes of device
  * memory; CSAW_ERR_BUDGET_TOO_SMALL when the resident state does not fit.
- * The edge list equals the in-memory csaw_sample result. */
+ * The edge list equals the in-memory csaw_sample result whatever the mode,
+ * schedule or variant.  opts NULL: batched, heuristic schedule. */
+int csaw_stream_sample_opts(csaw_host_graph *g, const csaw_request *req, int64_t budget_bytes,
+                            int32_t n_partitions, const csaw_stream_opts *opts, csaw_edges **out,
+                            csaw_stream_stats *stats);
+/* csaw_stream_sample_opts with opts NULL. */
 int csaw_stream_sample(csaw_host_graph *g, const csaw_request *req, int64_t budget_bytes,
                        int32_t n_partitions, csaw_edges **out, csaw_stream_stats *stats);
 /* Wraps host arrays as a csaw_edges result (copies them). */

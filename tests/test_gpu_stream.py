@@ -31,6 +31,74 @@ def test_stream_equals_in_memory(C, graphs, bias, parts):
     assert got["stats"].peak_device_bytes <= 1 << 30
 
 
+def _schedules(C):
+    yield "plain", dict(mode=C.MODE_PLAIN)
+    yield "heuristic", dict(mode=C.MODE_BATCHED, sched=C.SCHED_HEURISTIC)
+    for v in range(C.NUM_VARIANTS):
+        yield f"forced{v}", dict(mode=C.MODE_BATCHED, sched=C.SCHED_FORCED, variant=v)
+    yield "random", dict(mode=C.MODE_BATCHED, sched=C.SCHED_RANDOM, explore_key=5)
+    # an untrained model picks arbitrary valid variants
+    yield "mlp", dict(mode=C.MODE_BATCHED, sched=C.SCHED_MLP, mlp=C.Mlp(3))
+
+
+@pytest.mark.parametrize("bias", [0, 1, 3])
+def test_stream_schedules_equal_in_memory(C, graphs, bias):
+    g, hg, _ = graphs
+    seeds = g.seeds(1500, request=10 + bias)
+    for dedup in (1, 0):
+        want = C.sample(g, seeds, [15, 10, 5], bias=bias, dedup=dedup, request=2)
+        for name, kw in _schedules(C):
+            got = C.stream_sample(hg, seeds, [15, 10, 5], bias=bias, dedup=dedup, request=2, partitions=6,
+                                  budget_bytes=1 << 30, **kw)
+            same(got, want, f"bias {bias} dedup {dedup} {name}")
+            st = got["stats"]
+            vc = list(st.variant_chunks)
+            if name == "plain":
+                assert st.chunks == 0 and sum(vc) == 0
+            else:
+                assert st.chunks > 0 and sum(vc) <= st.chunks
+            if name == "forced0" and bias == 0:
+                # THREAD is valid for every uniform chunk with fanout <= 32
+                assert vc[0] == sum(vc)
+            if name in ("forced3", "forced4") and bias == 1:
+                assert vc[int(name[-1])] == sum(vc)
+
+
+def _star(hub_deg, extra, key):
+    """vertex 0 has hub_deg neighbours with 0..extra-1 further leaves each."""
+    rng = np.random.default_rng(key)
+    src, dst = [], []
+    nxt = hub_deg + 1
+    for i in range(1, hub_deg + 1):
+        src += [0, i]
+        dst += [i, 0]
+        for _ in range(int(rng.integers(0, extra))):
+            src += [i, nxt]
+            dst += [nxt, i]
+            nxt += 1
+    src, dst = np.array(src, np.int64), np.array(dst, np.int32)
+    o = np.lexsort((dst, src))
+    src, dst = src[o], dst[o]
+    off = np.zeros(nxt + 1, np.int64)
+    np.add.at(off, src + 1, 1)
+    return np.cumsum(off), dst
+
+
+@pytest.mark.parametrize("hub_deg", [1500, 9000, 20000])
+def test_stream_hub_schedules(C, hub_deg):
+    # integer-bias hubs above the workgroup CDF limits come from the
+    # per-partition hub pool; the hub sits in partition 0 of 3
+    off, adj = _star(hub_deg, 4, hub_deg)
+    g = C.Graph.from_csr(off, adj)
+    hg = C.HostGraph.from_csr(off, adj)
+    seeds = np.concatenate([np.zeros(48, np.int32), np.arange(1, 17, dtype=np.int32)])
+    for bias in (1, 3):
+        want = C.sample(g, seeds, [64, 2], bias=bias)
+        for name, kw in _schedules(C):
+            got = C.stream_sample(hg, seeds, [64, 2], bias=bias, partitions=3, budget_bytes=1 << 28, **kw)
+            same(got, want, f"hub {hub_deg} bias {bias} {name}")
+
+
 def test_stream_matches_oracle(C, O, graphs):
     g, hg, og = graphs
     seeds = g.seeds(500, request=3)
@@ -51,6 +119,15 @@ def test_stream_budget_too_small(C, graphs):
     with pytest.raises(C.CsawError) as ei:
         C.stream_sample(hg, g.seeds(100), [5, 5], budget_bytes=1 << 20, partitions=4)
     assert ei.value.status == 4
+
+
+def test_stream_rejects_bad_opts(C, graphs):
+    g, hg, _ = graphs
+    for kw in (dict(mode=5), dict(sched=9), dict(sched=C.SCHED_FORCED, variant=C.NUM_VARIANTS),
+               dict(sched=C.SCHED_MLP, mlp=None)):
+        with pytest.raises(C.CsawError) as ei:
+            C.stream_sample(hg, g.seeds(10), [5], budget_bytes=1 << 28, **kw)
+        assert ei.value.status == 1
 
 
 def test_stream_rejects_weights(C, graphs):
