@@ -204,21 +204,57 @@ def bench_walk(a, C, O):
          {"workload": "RMAT-22 ef16, FP32 weights, 1,048,576 walkers x 80", "steps": int(steps)}, "f32", roof, cpu)
 
 
-def bench_stream(a, C, O):
+def get_nocache_model(C, CM, retrain):
+    """Cost model of the kernels the streaming sampler runs (integer CDFs
+    built per entry, no per-graph cache): trained on in-memory runs of an
+    RMAT-22 graph with the streaming-compatible request mix, with the
+    held-out regret of the learned and the fixed schedules in its meta."""
+    path = CM.NOCACHE_MODEL
+    if os.path.exists(path) and not retrain:
+        return CM.load(C, path)
+    t0 = time.time()
+    g = C.Graph.rmat(22, 16)
+    data = CM.collect(C, g, CM.stream_specs(C, g, 64, mix_key=7), cdf_cache=False)
+    m, loss = CM.train(C, data)
+    held = CM.collect(C, g, CM.stream_specs(C, g, 32, mix_key=42), cdf_cache=False)
+    meta = {"graph": "RMAT-22 ef16", "train_mix_key": 7, "train_requests": 64, "chunks": int(len(data["keys"])),
+            "final_loss": loss, "held_out_mix_key": 42, "held_out_requests": 32,
+            "held_out_regret": CM.regret(held, m), "train_seconds": time.time() - t0}
+    g.close()
+    CM.save(m, path, meta)
+    return m, meta
+
+
+def bench_stream(a, C, O, CM):
     """C5: RMAT-26 ef16 in pinned host memory, 4 GB device budget, 16 partitions,
-    65,536 seeds, fanouts {15,10,5}, degree bias."""
+    65,536 seeds, fanouts {15,10,5}, degree bias; each partition's chunks are
+    run by the variant the learned cost model picks, the fixed heuristic and
+    the plain one-wave-per-entry expansion are timed beside it."""
+    mlp, meta = get_nocache_model(C, CM, a.retrain)
     t0 = time.time()
     hg = C.HostGraph.rmat(26, 16)
     log(f"host graph V={hg.V} E={hg.E} maxdeg={hg.max_degree} ({time.time() - t0:.1f}s)")
     seeds = C.seeds_v(hg.V, 65536)
-    res = None
-    ms = []
-    for i in range(a.warmup + a.steps):
-        res = C.stream_sample(hg, seeds, [15, 10, 5], bias=C.BIAS_DEGREE, budget_bytes=4 << 30, partitions=16)
-        if i >= a.warmup:
-            ms.append(res["stats"].device_ms)
+    kw = dict(bias=C.BIAS_DEGREE, budget_bytes=4 << 30, partitions=16)
+
+    def runs(n, warm, **o):
+        res, ms, ems = None, [], []
+        for i in range(warm + n):
+            res = C.stream_sample(hg, seeds, [15, 10, 5], **kw, **o)
+            if i >= warm:
+                ms.append(res["stats"].device_ms)
+                ems.append(res["stats"].expand_ms)
+        return res, float(np.median(ms)), float(np.median(ems))
+
+    res, dms, ems = runs(a.steps, a.warmup, mode=C.MODE_BATCHED, sched=C.SCHED_MLP, mlp=mlp)
+    nside = max(1, min(a.steps, 3))
+    heur, hms, hems = runs(nside, 1, mode=C.MODE_BATCHED, sched=C.SCHED_HEURISTIC)
+    plain, pms, pems = runs(nside, 1, mode=C.MODE_PLAIN)
+    keys = ("src", "dst", "smp", "hop_off")
+    equal = all(np.array_equal(res[k], heur[k]) and np.array_equal(res[k], plain[k]) for k in keys)
     st = res["stats"]
-    dms = float(np.median(ms))
+    log(f"stream: mlp {dms:.1f} ms (expand {ems:.1f})  heuristic {hms:.1f} (expand {hems:.1f})  "
+        f"plain {pms:.1f} (expand {pems:.1f})  equal {equal}")
     cpu = None
     if not a.no_cpu:
         off, adj = hg.csr_views()
@@ -234,9 +270,16 @@ def bench_stream(a, C, O):
                                          f"streamed GPU edges: {okh}"}
     line("SEPS streaming (C5)", st.edges / (dms * 1e-3), "edges/s", a.steps, a.warmup, dms,
          {"workload": "RMAT-26 ef16 host-resident, 4 GB budget, 16 partitions, 65,536 seeds, {15,10,5}, degree",
-          "edges": int(st.edges), "h2d_bytes": st.h2d_bytes, "peak_device_bytes": int(st.peak_device_bytes),
-          "slice_bytes": int(st.slice_bytes)}, "u64", None, cpu,
-         h2d_gbs=st.h2d_bytes / (dms * 1e-3) / 1e9)
+          "schedule": "learned cost model per chunk", "edges": int(st.edges), "h2d_bytes": st.h2d_bytes,
+          "peak_device_bytes": int(st.peak_device_bytes), "slice_bytes": int(st.slice_bytes)}, "u64", None, cpu,
+         h2d_gbs=st.h2d_bytes / (dms * 1e-3) / 1e9,
+         schedules={"mlp": {"ms": dms, "expand_ms": ems, "variant_chunks": list(st.variant_chunks),
+                            "chunks": int(st.chunks)},
+                    "heuristic": {"ms": hms, "expand_ms": hems,
+                                  "variant_chunks": list(heur["stats"].variant_chunks),
+                                  "chunks": int(heur["stats"].chunks)},
+                    "plain": {"ms": pms, "expand_ms": pems}},
+         schedules_bit_equal=bool(equal), cost_model=meta)
 
 
 def main():
@@ -303,7 +346,7 @@ def main():
         elif a.workload == "c3":
             bench_walk(a, C, O)
         else:
-            bench_stream(a, C, O)
+            bench_stream(a, C, O, CM)
         return
 
     t0 = time.time()

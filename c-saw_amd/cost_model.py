@@ -19,6 +19,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 MODEL_DIR = os.path.join(HERE, "models")
 DEFAULT_MODEL = os.path.join(MODEL_DIR, "cost_model_gfx950.json")
+# trained on runs without the integer CDF cache: the kernels the streaming
+# sampler runs (it builds integer CDFs per entry, hubs per partition)
+NOCACHE_MODEL = os.path.join(MODEL_DIR, "cost_model_gfx950_nocache.json")
 # concurrent chunk workers per CU of each variant's kernel (LaunchCfg
 # defaults): THREAD and WAVE kernels claim chunks per wave (4 waves per
 # workgroup), BLOCK per workgroup
@@ -55,6 +58,19 @@ def collect(C, g, specs, cdf_cache=True, wg_per_cu=None):
         T[pos[ok], v] = lt[ok] - np.log2(wg[v])
         M[pos[ok], v] = 1.0
     return {"keys": keys, "X": X, "T": T, "M": M, "heur": heur}
+
+
+def stream_specs(C, g, n, mix_key, lo=64, hi=65536):
+    """A request mix the streaming sampler accepts: the C4 mix with FP32
+    weight bias replaced by degree bias."""
+    out = []
+    for i, r in enumerate(C.request_mix(n, lo, hi, mix_key)):
+        r = dict(r)
+        if r["bias"] == C.BIAS_WEIGHT:
+            r["bias"] = C.BIAS_DEGREE
+        r["seeds"] = g.seeds(r["samples"], key=42, request=i)
+        out.append(r)
+    return out
 
 
 def train(C, data, epochs=60, batch=256, lr=0.05, init_key=1, shuffle_key=7):

@@ -76,6 +76,8 @@ _sig("csaw_graph_create", [P, ctypes.c_int, P, P, i64, i64, P])
 _sig("csaw_graph_create_rmat", [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, u64,
                                 ctypes.c_int, P])
 _sig("csaw_graph_load_bin", [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, P])
+_sig("csaw_csr_write_bin", [ctypes.c_char_p, ctypes.c_char_p, P, P, i64, i64])
+_sig("csaw_snap_to_bin", [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, P, P])
 _sig("csaw_graph_info", [P, P, P, P])
 _sig("csaw_graph_copy_csr", [P, P, P, P])
 _sig("csaw_graph_destroy", [P])
@@ -545,3 +547,19 @@ def seeds_v(V, n, key=42, request=0):
     out = np.zeros(max(n, 1), np.int32)
     _check(_L.csaw_seeds_v(V, key, request, n, _p(out)), "seeds_v")
     return out[:n]
+
+
+def write_bin(beg_file, csr_file, off, adj):
+    """Writes a CSR as the reference's beg.bin / csr.bin (int64 offsets and ids)."""
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    adj = np.ascontiguousarray(adj, dtype=np.int32)
+    _check(_L.csaw_csr_write_bin(beg_file.encode(), csr_file.encode(), _p(off), _p(adj), len(off) - 1, len(adj)),
+           "csr_write_bin")
+
+
+def snap_to_bin(snap_file, beg_file, csr_file, symmetrize=True):
+    """SNAP text edge list -> beg.bin / csr.bin; returns (V, E)."""
+    V, E = i64(), i64()
+    _check(_L.csaw_snap_to_bin(snap_file.encode(), 1 if symmetrize else 0, beg_file.encode(), csr_file.encode(),
+                               ctypes.byref(V), ctypes.byref(E)), "snap_to_bin")
+    return V.value, E.value

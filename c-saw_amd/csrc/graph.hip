@@ -133,7 +133,6 @@ __global__ void k_seeds(uint32_t k0, uint32_t k1, uint32_t request, int64_t V, i
 // Hub cache, split across workgroups: each workgroup scans one segment of
 // one hub's neighbour list (kHubSeg entries) and writes the segment-local
 // inclusive CDF plus the segment total; a second pass adds segment offsets.
-constexpr int kHubSeg = 4096;
 constexpr int kHubThreads = 256;
 
 template <int KIND>  // 0 degree, 1 user hook
@@ -324,21 +323,30 @@ int hub_cdf_build(const GraphDev& G, const csaw_graph_view& hv, int kind, const 
   CSAW_HIP(hipMemcpyAsync(d_seg_idx, seg_idx.data(), sizeof(int32_t) * n_seg, hipMemcpyHostToDevice, stream));
   CSAW_HIP(hipMemcpyAsync(d_seg_first, seg_first.data(), sizeof(int64_t) * seg_first.size(), hipMemcpyHostToDevice,
                           stream));
-  const int grid = (int)std::min<int64_t>(n_seg, (int64_t)num_cu * 8);
-  if (kind == 0)
-    k_hub_segments<0><<<grid, kHubThreads, 0, stream>>>(G, hv, d_seg_hub, d_seg_idx, n_seg, out, d_seg_tot);
-  else
-    k_hub_segments<1><<<grid, kHubThreads, 0, stream>>>(G, hv, d_seg_hub, d_seg_idx, n_seg, out, d_seg_tot);
-  CSAW_LAUNCH_CHECK();
-  k_hub_seg_offsets<<<grid_for(n_hubs, 64, num_cu), 64, 0, stream>>>(d_seg_first, n_hubs, d_seg_tot);
-  CSAW_LAUNCH_CHECK();
-  k_hub_add<<<grid, kHubThreads, 0, stream>>>(G, d_seg_hub, d_seg_idx, n_seg, d_seg_tot, out);
-  CSAW_LAUNCH_CHECK();
+  CSAW_TRY(hub_cdf_segments(G, hv, kind, n_hubs, d_seg_hub, d_seg_idx, n_seg, d_seg_first, d_seg_tot, out, stream,
+                            num_cu));
   CSAW_HIP(hipStreamSynchronize(stream));
   CSAW_HIP(hipFree(d_seg_hub));
   CSAW_HIP(hipFree(d_seg_idx));
   CSAW_HIP(hipFree(d_seg_first));
   CSAW_HIP(hipFree(d_seg_tot));
+  return CSAW_OK;
+}
+
+int hub_cdf_segments(const GraphDev& G, const csaw_graph_view& hv, int kind, int32_t n_hubs, const int32_t* seg_hub,
+                     const int32_t* seg_idx, int64_t n_seg, const int64_t* seg_first, uint64_t* seg_tot,
+                     uint64_t* out, hipStream_t stream, int num_cu) {
+  if (n_hubs <= 0 || n_seg <= 0) return CSAW_OK;
+  const int grid = (int)std::min<int64_t>(n_seg, (int64_t)num_cu * 8);
+  if (kind == 0)
+    k_hub_segments<0><<<grid, kHubThreads, 0, stream>>>(G, hv, seg_hub, seg_idx, n_seg, out, seg_tot);
+  else
+    k_hub_segments<1><<<grid, kHubThreads, 0, stream>>>(G, hv, seg_hub, seg_idx, n_seg, out, seg_tot);
+  CSAW_LAUNCH_CHECK();
+  k_hub_seg_offsets<<<grid_for(n_hubs, 64, num_cu), 64, 0, stream>>>(seg_first, n_hubs, seg_tot);
+  CSAW_LAUNCH_CHECK();
+  k_hub_add<<<grid, kHubThreads, 0, stream>>>(G, seg_hub, seg_idx, n_seg, seg_tot, out);
+  CSAW_LAUNCH_CHECK();
   return CSAW_OK;
 }
 

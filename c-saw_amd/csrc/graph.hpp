@@ -43,6 +43,14 @@ int graph_prepare_int_cache(csaw_graph* g, int32_t bias);
 int graph_prepare_index(csaw_graph* g, int32_t bias);
 int hub_cdf_build(const GraphDev& G, const csaw_graph_view& hv, int kind, const std::vector<int64_t>& pool,
                   uint64_t* out, hipStream_t stream, int num_cu);
+// Hub CDFs from a prepared segment list (no allocation, no synchronisation):
+// segment s covers entries [seg_idx[s] * kHubSeg, +kHubSeg) of hub
+// seg_hub[s] (vertex G.hub_vid[h], CDF written at out[G.hub_pool[h] + i]);
+// seg_first[h] .. seg_first[h+1] are hub h's segments; seg_tot is scratch.
+constexpr int kHubSeg = 4096;
+int hub_cdf_segments(const GraphDev& G, const csaw_graph_view& hv, int kind, int32_t n_hubs, const int32_t* seg_hub,
+                     const int32_t* seg_idx, int64_t n_seg, const int64_t* seg_first, uint64_t* seg_tot,
+                     uint64_t* out, hipStream_t stream, int num_cu);
 int rmat_generate_device(int scale, int edge_factor, double a, double b, double c, uint64_t key, hipStream_t stream,
                          int num_cu, int64_t* V, int64_t* E, int64_t** d_off, int32_t** d_adj);
 int graph_alloc_common(csaw_graph* g);

@@ -9,8 +9,11 @@
 //    frontier is binned by partition and each partition's adjacency slice is
 //    copied on a second stream into one of two device buffers while the
 //    previous partition expands (double buffering);
-//  - integer-bias hubs of the partition's frontier get their CDF built once
-//    per (hop, partition) into a pool, split across workgroups;
+//  - for integer biases, every distinct frontier vertex of the partition gets
+//    its inclusive CDF built once per (hop, partition), at its own position
+//    in a slice-shaped array (one wave per vertex; vertices above kHubSeg
+//    neighbours split across workgroups), so every draw is a search of that
+//    array instead of a pass over the neighbour list per frontier entry;
 //  - output positions come from the same global prefix sums as the
 //    in-memory run, so the edge list is bit-identical to it.
 // In batched mode each partition's entries are binned into chunks by degree
@@ -41,10 +44,12 @@ __global__ void k_stream_degree(const int64_t* __restrict__ off, int64_t V, int3
     deg[v] = (int32_t)(off[v + 1] - off[v]);
 }
 
-// partition id per frontier entry, per-partition counts, hub marks
+// partition id per frontier entry, per-partition counts; for integer biases
+// the vertices whose entries draw (d > 0, F > 0, not a take-all copy) are
+// marked in fbits (up to kHubSeg neighbours) or hubbits (more)
 __global__ void k_part_bin(GraphDev G, const ReqDev* __restrict__ R, HopDev H, const int32_t* __restrict__ vb,
                            int P, uint8_t* __restrict__ pid, uint32_t* __restrict__ cnt,
-                           uint32_t* __restrict__ hubbits) {
+                           uint32_t* __restrict__ fbits, uint32_t* __restrict__ hubbits) {
   const int64_t n = *H.n_ptr;
   const ReqDev& r = R[0];
   const int32_t F = r.fanout[H.hop];
@@ -59,7 +64,66 @@ __global__ void k_part_bin(GraphDev G, const ReqDev* __restrict__ R, HopDev H, c
     pid[e] = (uint8_t)lo;
     atomicAdd(&cnt[lo], 1u);
     const int32_t d = G.deg[v];
-    if (int_bias && d > kHubMin && d > F) atomicOr(&hubbits[v >> 5], 1u << (v & 31));
+    if (int_bias && d > 0 && F > 0 && !(r.dedup && d <= F)) {
+      uint32_t* bits = d > kHubSeg ? hubbits : fbits;
+      const uint32_t bit = 1u << (v & 31);
+      if (!(bits[v >> 5] & bit)) atomicOr(&bits[v >> 5], bit);
+    }
+  }
+}
+
+// The vertices marked in fbits within [v0, v1), compacted (in no particular
+// order) into list[0 .. *n).
+__global__ __launch_bounds__(256) void k_mark_list(const uint32_t* __restrict__ fbits, int32_t v0, int32_t v1,
+                                                   int32_t* __restrict__ list, uint32_t* __restrict__ n) {
+  const int64_t w0 = v0 >> 5, w1 = ((int64_t)v1 + 31) >> 5;
+  for (int64_t wb = w0 + blockIdx.x * (int64_t)blockDim.x; wb < w1; wb += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t wi = wb + threadIdx.x;
+    uint32_t m = 0;
+    if (wi < w1) {
+      m = fbits[wi];
+      const int64_t lo = (int64_t)v0 - wi * 32, hi = (int64_t)v1 - wi * 32;  // keep bits lo .. hi-1
+      if (lo > 0) m &= lo >= 32 ? 0u : ~0u << lo;
+      if (hi < 32) m &= hi <= 0 ? 0u : ~0u >> (32 - hi);
+    }
+    const uint32_t c = (uint32_t)__popc(m);
+    const uint32_t incl = wave_incl_scan_u32(c);
+    uint32_t base = 0;
+    if (lane_id() == kWave - 1 && incl) base = atomicAdd(n, incl);
+    base = readlane_u(base, kWave - 1) + incl - c;
+    while (m) {
+      const int b = __builtin_ctz(m);
+      m &= m - 1;
+      list[base++] = (int32_t)(wi * 32 + b);
+    }
+  }
+}
+
+// Inclusive integer CDFs of the listed vertices (up to kHubSeg neighbours),
+// one wave per vertex, written at their slice positions: C[o - a + i] for
+// entry i of vertex v (o = off[v]).
+template <int KIND>  // 0 degree, 1 user hook
+__global__ __launch_bounds__(256) void k_frontier_cdf(GraphDev G, csaw_graph_view hv, const int32_t* __restrict__ list,
+                                                      const uint32_t* __restrict__ n_ptr, int64_t a,
+                                                      uint64_t* __restrict__ C) {
+  const int lane = lane_id();
+  const int64_t n = *n_ptr;
+  for (int64_t k = blockIdx.x * 4 + threadIdx.x / kWave; k < n; k += (int64_t)gridDim.x * 4) {
+    const int32_t v = list[k];
+    const int64_t o = G.off[v];
+    const int32_t d = (int32_t)(G.off[v + 1] - o);
+    uint64_t acc = 0;
+    for (int i0 = 0; i0 < d; i0 += kWave) {
+      const int i = i0 + lane;
+      uint64_t w = 0;
+      if (i < d) {
+        const int32_t u = G.adj[o + i];
+        w = KIND == 0 ? (uint64_t)G.deg[u] : (uint64_t)EdgeBias(u, &hv);
+      }
+      const uint64_t sc = wave_incl_scan_u64(w);
+      if (i < d) C[o - a + i] = acc + sc;
+      acc += readlane_u64(sc, kWave - 1);
+    }
   }
 }
 
@@ -233,7 +297,7 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
   for (int p = 0; p < P; ++p) {
     slice_max = std::max(slice_max, hg->off[vb[p + 1]] - hg->off[vb[p]]);
     int64_t nh = 0;
-    for (int32_t v = vb[p]; v < vb[p + 1]; ++v) nh += (hg->off[v + 1] - hg->off[v]) > kHubMin;
+    for (int32_t v = vb[p]; v < vb[p + 1]; ++v) nh += (hg->off[v + 1] - hg->off[v]) > kHubSeg;
     hubs_max = std::max(hubs_max, nh);
   }
   // per-hop capacities
@@ -278,15 +342,27 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
     int32_t* slice[2];
     CSAW_TRY(B.alloc(&slice[0], slice_max));
     CSAW_TRY(B.alloc(&slice[1], slice_max));
+    // integer biases: the partition's frontier CDFs at slice positions, the
+    // marks, and the hub segment lists
     uint64_t* pool = nullptr;
-    int32_t* d_hub_vid = nullptr;
-    int64_t* d_hub_pool = nullptr;
-    uint32_t* d_hubbits = nullptr;
+    int32_t *d_hub_vid = nullptr, *d_seg_hub = nullptr, *d_seg_idx = nullptr;
+    int64_t *d_hub_pos = nullptr, *d_seg_first = nullptr;
+    uint64_t* d_seg_tot = nullptr;
+    uint32_t *d_hubbits = nullptr, *d_fbits = nullptr, *d_fcount = nullptr;
+    int32_t* d_fvlist = nullptr;
+    const int64_t seg_max = slice_max / kHubSeg + hubs_max + 1;
     if (int_bias) {
       CSAW_TRY(B.alloc(&pool, slice_max));
       CSAW_TRY(B.alloc(&d_hubbits, (V + 31) / 32));
+      CSAW_TRY(B.alloc(&d_fbits, (V + 31) / 32));
+      CSAW_TRY(B.alloc(&d_fcount, 1));
+      CSAW_TRY(B.alloc(&d_fvlist, mx));  // distinct vertices of one partition's frontier
       CSAW_TRY(B.alloc(&d_hub_vid, hubs_max));
-      CSAW_TRY(B.alloc(&d_hub_pool, hubs_max + 1));
+      CSAW_TRY(B.alloc(&d_hub_pos, hubs_max));
+      CSAW_TRY(B.alloc(&d_seg_first, hubs_max + 1));
+      CSAW_TRY(B.alloc(&d_seg_hub, seg_max));
+      CSAW_TRY(B.alloc(&d_seg_idx, seg_max));
+      CSAW_TRY(B.alloc(&d_seg_tot, seg_max));
     }
     // request / frontier / outputs
     ReqDev rd;
@@ -420,8 +496,11 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
       CSAW_TRY(launch_scans(L, H, d_tot + h, nmax[h]));
       // bin the frontier by partition
       CSAW_TRY(launch_fill(L, d_cnt, 0u, 3 * P));
-      if (int_bias) CSAW_TRY(launch_fill(L, d_hubbits, 0u, (V + 31) / 32));
-      k_part_bin<<<grid_of(nmax[h], num_cu), 256, 0, cs>>>(G, d_req, H, d_vb, P, d_pid, d_cnt, d_hubbits);
+      if (int_bias) {
+        CSAW_TRY(launch_fill(L, d_hubbits, 0u, (V + 31) / 32));
+        CSAW_TRY(launch_fill(L, d_fbits, 0u, (V + 31) / 32));
+      }
+      k_part_bin<<<grid_of(nmax[h], num_cu), 256, 0, cs>>>(G, d_req, H, d_vb, P, d_pid, d_cnt, d_fbits, d_hubbits);
       CSAW_LAUNCH_CHECK();
       std::vector<uint32_t> cnt(P), poff(P + 1, 0);
       CSAW_HIP(hipMemcpyAsync(cnt.data(), d_cnt, sizeof(uint32_t) * P, hipMemcpyDeviceToHost, cs));
@@ -486,7 +565,7 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
           A.queues = d_queues;
           A.queue_cap = chunk_cap;
           A.chosen = nullptr;
-          A.int_cache = 0;
+          A.int_cache = int_bias ? 1 : 0;  // every drawing vertex has its CDF
           A.mlp.w = d_mlp;
           A.mlp_w = d_mlp;
           const int64_t nblk_p = ((int64_t)cnt[p] + kBlockEntries - 1) / kBlockEntries;
@@ -501,19 +580,49 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
         Gp.adj = slice[buf] - hg->off[vb[p]];  // global offsets index the slice
         csaw_graph_view hvp = hv;
         hvp.adj_list = Gp.adj;
-        if (int_bias && !hubs[p].empty()) {
-          std::vector<int64_t> hp(hubs[p].size() + 1, 0);
-          for (size_t i = 0; i < hubs[p].size(); ++i)
-            hp[i + 1] = hp[i] + (hg->off[hubs[p][i] + 1] - hg->off[hubs[p][i]]);
-          CSAW_HIP(hipMemcpyAsync(d_hub_vid, hubs[p].data(), sizeof(int32_t) * hubs[p].size(), hipMemcpyHostToDevice, cs));
-          CSAW_HIP(hipMemcpyAsync(d_hub_pool, hp.data(), sizeof(int64_t) * hp.size(), hipMemcpyHostToDevice, cs));
-          Gp.hub_vid = d_hub_vid;
-          Gp.hub_pool = d_hub_pool;
-          Gp.n_hubs = (int32_t)hubs[p].size();
-          Gp.hub_cdf[0] = Gp.hub_cdf[1] = pool;
-          CSAW_TRY(hub_cdf_build(Gp, hvp, r.bias == CSAW_BIAS_USER ? 1 : 0, hp, pool, cs, num_cu));
-        } else {
-          Gp.n_hubs = 0;
+        Gp.n_hubs = 0;
+        std::vector<int32_t> seg_hub, seg_idx;
+        std::vector<int64_t> seg_first, hub_pos;
+        if (int_bias) {
+          // CDFs of the partition's distinct frontier vertices at their slice
+          // positions; every draw of this partition then searches them
+          const int ck = r.bias == CSAW_BIAS_USER ? 1 : 0;
+          const int64_t a = hg->off[vb[p]];
+          const int64_t words = ((int64_t)vb[p + 1] + 31) / 32 - vb[p] / 32;
+          CSAW_TRY(launch_fill(L, d_fcount, 0u, 1));
+          k_mark_list<<<grid_of(words, num_cu), 256, 0, cs>>>(d_fbits, vb[p], vb[p + 1], d_fvlist, d_fcount);
+          CSAW_LAUNCH_CHECK();
+          const int gcdf = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)cnt[p] + 3) / 4, (int64_t)num_cu * 8));
+          if (ck == 0) k_frontier_cdf<0><<<gcdf, 256, 0, cs>>>(Gp, hvp, d_fvlist, d_fcount, a, pool);
+          else k_frontier_cdf<1><<<gcdf, 256, 0, cs>>>(Gp, hvp, d_fvlist, d_fcount, a, pool);
+          CSAW_LAUNCH_CHECK();
+          if (!hubs[p].empty()) {
+            seg_first.push_back(0);
+            for (size_t i = 0; i < hubs[p].size(); ++i) {
+              const int32_t v = hubs[p][i];
+              const int64_t d = hg->off[v + 1] - hg->off[v];
+              hub_pos.push_back(hg->off[v] - a);
+              for (int64_t sg = 0; sg * kHubSeg < d; ++sg) {
+                seg_hub.push_back((int32_t)i);
+                seg_idx.push_back((int32_t)sg);
+              }
+              seg_first.push_back((int64_t)seg_hub.size());
+            }
+            const int32_t nh = (int32_t)hubs[p].size();
+            const int64_t ns = (int64_t)seg_hub.size();
+            CSAW_HIP(hipMemcpyAsync(d_hub_vid, hubs[p].data(), sizeof(int32_t) * nh, hipMemcpyHostToDevice, cs));
+            CSAW_HIP(hipMemcpyAsync(d_hub_pos, hub_pos.data(), sizeof(int64_t) * nh, hipMemcpyHostToDevice, cs));
+            CSAW_HIP(hipMemcpyAsync(d_seg_first, seg_first.data(), sizeof(int64_t) * (nh + 1), hipMemcpyHostToDevice, cs));
+            CSAW_HIP(hipMemcpyAsync(d_seg_hub, seg_hub.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, cs));
+            CSAW_HIP(hipMemcpyAsync(d_seg_idx, seg_idx.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, cs));
+            GraphDev Gh = Gp;
+            Gh.hub_vid = d_hub_vid;
+            Gh.hub_pool = d_hub_pos;
+            Gh.n_hubs = nh;
+            CSAW_TRY(hub_cdf_segments(Gh, hvp, ck, nh, d_seg_hub, d_seg_idx, ns, d_seg_first, d_seg_tot, pool, cs,
+                                      num_cu));
+          }
+          Gp.int_cdf[ck] = pool - a;  // global offsets index the slice-shaped array
         }
         if (batched) {
           ExpandArgs X;
@@ -522,14 +631,14 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
           X.qlen = d_ctr + kCtrQlen;
           X.qhead = d_ctr + kCtrQhead;
           X.log = nullptr;
-          X.int_cache = 0;
+          X.int_cache = int_bias ? 1 : 0;
           CSAW_TRY(launch_expand_chunks(L, Gp, hvp, d_req, d_gs2req, H, X, kinds_present, nullptr));
         } else {
           CSAW_TRY(launch_expand_list(L, kind, Gp, hvp, d_req, d_gs2req, H, d_list + poff[p], (int64_t)cnt[p]));
         }
         CSAW_HIP(hipEventRecord(ex1, cs));
         CSAW_HIP(hipEventRecord(freed[buf], cs));
-        if (int_bias && !hubs[p].empty()) CSAW_HIP(hipStreamSynchronize(cs));  // hub list buffers are reused
+        if (!seg_hub.empty()) CSAW_HIP(hipStreamSynchronize(cs));  // hub list buffers are reused
       }
     }
     CSAW_HIP(hipEventRecord(t1, cs));

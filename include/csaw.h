@@ -94,6 +94,16 @@ int csaw_graph_create_rmat(int scale, int edge_factor, double a, double b, doubl
  * E int64 ids; graph.hpp:18-74) and uploads it; weight_file may be NULL. */
 int csaw_graph_load_bin(const char *beg_file, const char *csr_file, const char *weight_file,
                         csaw_graph **out);
+/* Writes a host CSR in that format (int64 offsets and ids). */
+int csaw_csr_write_bin(const char *beg_file, const char *csr_file, const int64_t *offsets,
+                       const int32_t *indices, int64_t V, int64_t E);
+/* Converts a SNAP text edge list ('#' / '%' comment lines, "src dst" per
+ * line, extra columns ignored) to beg.bin / csr.bin: ids kept, V = largest
+ * id + 1, optional symmetrisation, self loops and repeated edges dropped,
+ * neighbour lists sorted (the reference relied on an external converter,
+ * README.md:14-22). */
+int csaw_snap_to_bin(const char *snap_file, int symmetrize, const char *beg_file, const char *csr_file,
+                     int64_t *V, int64_t *E);
 int csaw_graph_info(const csaw_graph *g, int64_t *V, int64_t *E, int64_t *max_degree);
 /* Copies the CSR back to host buffers (any may be NULL). */
 int csaw_graph_copy_csr(const csaw_graph *g, int64_t *offsets, int32_t *indices, float *weights);

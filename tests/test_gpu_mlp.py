@@ -71,3 +71,16 @@ def test_learned_scheduler_beats_heuristic(C):
     print("held-out regret", r)
     assert r["chunks"] > 100
     assert r["mlp"] < r["heuristic"], r
+
+
+def test_learned_scheduler_without_cdf_cache(C):
+    # the kernels the streaming sampler runs: integer CDFs built per entry
+    import cost_model as CM
+    g = C.Graph.rmat(18, 16)
+    train = CM.collect(C, g, CM.stream_specs(C, g, 48, 7, hi=8192), cdf_cache=False)
+    test = CM.collect(C, g, CM.stream_specs(C, g, 48, 42, hi=8192), cdf_cache=False)
+    mlp, loss = CM.train(C, train)
+    r = CM.regret(test, mlp)
+    print("held-out regret (no CDF cache)", r)
+    assert r["chunks"] > 100
+    assert r["mlp"] < r["heuristic"], r
