@@ -237,19 +237,26 @@ def bench_stream(a, C, O, CM):
     seeds = C.seeds_v(hg.V, 65536)
     kw = dict(bias=C.BIAS_DEGREE, budget_bytes=4 << 30, partitions=16)
 
-    def runs(n, warm, **o):
-        res, ms, ems = None, [], []
-        for i in range(warm + n):
-            res = C.stream_sample(hg, seeds, [15, 10, 5], **kw, **o)
-            if i >= warm:
-                ms.append(res["stats"].device_ms)
-                ems.append(res["stats"].expand_ms)
-        return res, float(np.median(ms)), float(np.median(ems))
-
-    res, dms, ems = runs(a.steps, a.warmup, mode=C.MODE_BATCHED, sched=C.SCHED_MLP, mlp=mlp)
-    nside = max(1, min(a.steps, 3))
-    heur, hms, hems = runs(nside, 1, mode=C.MODE_BATCHED, sched=C.SCHED_HEURISTIC)
-    plain, pms, pems = runs(nside, 1, mode=C.MODE_PLAIN)
+    # the three schedules take turns in every round (so that drift in the
+    # host link's speed does not favour one of them); medians per schedule
+    scheds = {"mlp": dict(mode=C.MODE_BATCHED, sched=C.SCHED_MLP, mlp=mlp),
+              "heuristic": dict(mode=C.MODE_BATCHED, sched=C.SCHED_HEURISTIC),
+              "plain": dict(mode=C.MODE_PLAIN)}
+    last, ms, ems = {}, {k: [] for k in scheds}, {k: [] for k in scheds}
+    names = list(scheds)
+    for i in range(a.warmup + a.steps):
+        for j in range(len(names)):
+            k = names[(i + j) % len(names)]
+            last[k] = C.stream_sample(hg, seeds, [15, 10, 5], **kw, **scheds[k])
+            if i >= a.warmup:
+                ms[k].append(last[k]["stats"].device_ms)
+                ems[k].append(last[k]["stats"].expand_ms)
+    log("per-run ms", {k: [round(x, 1) for x in v] for k, v in ms.items()})
+    res, heur, plain = last["mlp"], last["heuristic"], last["plain"]
+    dms, ems_ = float(np.median(ms["mlp"])), float(np.median(ems["mlp"]))
+    hms, hems = float(np.median(ms["heuristic"])), float(np.median(ems["heuristic"]))
+    pms, pems = float(np.median(ms["plain"])), float(np.median(ems["plain"]))
+    ems = ems_
     keys = ("src", "dst", "smp", "hop_off")
     equal = all(np.array_equal(res[k], heur[k]) and np.array_equal(res[k], plain[k]) for k in keys)
     st = res["stats"]
@@ -271,7 +278,8 @@ def bench_stream(a, C, O, CM):
     line("SEPS streaming (C5)", st.edges / (dms * 1e-3), "edges/s", a.steps, a.warmup, dms,
          {"workload": "RMAT-26 ef16 host-resident, 4 GB budget, 16 partitions, 65,536 seeds, {15,10,5}, degree",
           "schedule": "learned cost model per chunk", "edges": int(st.edges), "h2d_bytes": st.h2d_bytes,
-          "peak_device_bytes": int(st.peak_device_bytes), "slice_bytes": int(st.slice_bytes)}, "u64", None, cpu,
+          "peak_device_bytes": int(st.peak_device_bytes), "slice_bytes": int(st.slice_bytes),
+          "direct_partitions": int(st.direct_partitions), "reused_slices": int(st.reused_slices)}, "u64", None, cpu,
          h2d_gbs=st.h2d_bytes / (dms * 1e-3) / 1e9,
          schedules={"mlp": {"ms": dms, "expand_ms": ems, "variant_chunks": list(st.variant_chunks),
                             "chunks": int(st.chunks)},

@@ -18,6 +18,7 @@ BIAS_UNIFORM, BIAS_DEGREE, BIAS_WEIGHT, BIAS_USER = 0, 1, 2, 3
 VAR_THREAD, VAR_WAVE_BITMAP, VAR_WAVE_HASH, VAR_BLOCK_BITMAP, VAR_BLOCK_HASH = range(5)
 VARIANT_NAMES = ["thread", "wave_bitmap", "wave_hash", "block_bitmap", "block_hash"]
 SCHED_HEURISTIC, SCHED_MLP, SCHED_FORCED, SCHED_RANDOM = 0, 1, 2, 3
+SLICE_AUTO, SLICE_COPY, SLICE_DIRECT = 0, 1, 2
 MODE_PLAIN, MODE_BATCHED = 0, 1
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "INVALID_GRAPH", 3: "OOM", 4: "BUDGET_TOO_SMALL", 5: "HIP",
           6: "CAPACITY", 7: "NO_DEVICE", 8: "UNSUPPORTED", 9: "IO"}
@@ -435,11 +436,13 @@ class Mlp:
 class StreamStats(ctypes.Structure):
     _fields_ = [("edges", i64), ("device_ms", ctypes.c_double), ("h2d_bytes", ctypes.c_double),
                 ("partitions", i32), ("peak_device_bytes", i64), ("slice_bytes", i64),
-                ("expand_ms", ctypes.c_double), ("chunks", i64), ("variant_chunks", i64 * NUM_VARIANTS)]
+                ("expand_ms", ctypes.c_double), ("chunks", i64), ("variant_chunks", i64 * NUM_VARIANTS),
+                ("direct_partitions", i32), ("reused_slices", i32)]
 
 
 class StreamOpts(ctypes.Structure):
-    _fields_ = [("mode", i32), ("sched", i32), ("variant", i32), ("explore_key", u64), ("mlp", P), ("tuning", P)]
+    _fields_ = [("mode", i32), ("sched", i32), ("variant", i32), ("explore_key", u64), ("mlp", P), ("tuning", P),
+                ("slice_policy", i32)]
 
 
 _sig("csaw_host_graph_create_rmat", [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -500,14 +503,14 @@ class HostGraph:
 
 def stream_sample(hg, seeds, fanouts, bias=BIAS_DEGREE, dedup=1, key=42, request=0, budget_bytes=4 << 30,
                   partitions=16, mode=MODE_BATCHED, sched=SCHED_HEURISTIC, variant=0, mlp=None, explore_key=0,
-                  tuning=None):
+                  tuning=None, slice_policy=0):
     """Streams one request; mode / sched / variant / mlp choose how each
     partition's entries are expanded (the edge list is the same for all)."""
     reqs, keep = make_requests([{"seeds": seeds, "fanouts": fanouts, "bias": bias, "dedup": dedup, "key": key,
                                  "request": request}])
     e = P()
     st = StreamStats()
-    o = StreamOpts(mode=mode, sched=sched, variant=variant, explore_key=explore_key,
+    o = StreamOpts(mode=mode, sched=sched, variant=variant, explore_key=explore_key, slice_policy=slice_policy,
                    mlp=mlp.h if mlp is not None else None,
                    tuning=ctypes.cast(ctypes.byref(tuning), P) if tuning is not None else None)
     _check(_L.csaw_stream_sample_opts(hg.h, reqs, budget_bytes, partitions, ctypes.byref(o), ctypes.byref(e),

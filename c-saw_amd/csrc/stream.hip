@@ -44,12 +44,28 @@ __global__ void k_stream_degree(const int64_t* __restrict__ off, int64_t V, int3
     deg[v] = (int32_t)(off[v + 1] - off[v]);
 }
 
-// partition id per frontier entry, per-partition counts; for integer biases
-// the vertices whose entries draw (d > 0, F > 0, not a take-all copy) are
-// marked in fbits (up to kHubSeg neighbours) or hubbits (more)
-__global__ void k_part_bin(GraphDev G, const ReqDev* __restrict__ R, HopDev H, const int32_t* __restrict__ vb,
-                           int P, uint8_t* __restrict__ pid, uint32_t* __restrict__ cnt,
-                           uint32_t* __restrict__ fbits, uint32_t* __restrict__ hubbits) {
+// Host bytes one frontier entry would read if its partition's adjacency
+// were read in place over the bus instead of copied: a 64-byte request per
+// picked neighbour, plus the neighbour list itself when an integer CDF is
+// built from it (an upper bound: a vertex's CDF is built once per partition).
+constexpr int64_t kBusRead = 64;
+constexpr int64_t kDirectFactor = 2;  // direct reads when 2x that estimate < the slice
+
+// partition id per frontier entry, per-partition counts and direct-read byte
+// estimates; for integer biases the vertices whose entries draw (d > 0,
+// F > 0, not a take-all copy) are marked in fbits (up to kHubSeg neighbours)
+// or hubbits (more)
+__global__ __launch_bounds__(256) void k_part_bin(GraphDev G, const ReqDev* __restrict__ R, HopDev H,
+                                                  const int32_t* __restrict__ vb, int P, uint8_t* __restrict__ pid,
+                                                  uint32_t* __restrict__ cnt, unsigned long long* __restrict__ pbytes,
+                                                  uint32_t* __restrict__ fbits, uint32_t* __restrict__ hubbits) {
+  __shared__ uint32_t lcnt[256];
+  __shared__ unsigned long long lbytes[256];
+  for (int i = threadIdx.x; i < P; i += blockDim.x) {
+    lcnt[i] = 0;
+    lbytes[i] = 0;
+  }
+  __syncthreads();
   const int64_t n = *H.n_ptr;
   const ReqDev& r = R[0];
   const int32_t F = r.fanout[H.hop];
@@ -62,13 +78,22 @@ __global__ void k_part_bin(GraphDev G, const ReqDev* __restrict__ R, HopDev H, c
       if (vb[mid] <= v) lo = mid; else hi = mid;
     }
     pid[e] = (uint8_t)lo;
-    atomicAdd(&cnt[lo], 1u);
+    atomicAdd(&lcnt[lo], 1u);
     const int32_t d = G.deg[v];
-    if (int_bias && d > 0 && F > 0 && !(r.dedup && d <= F)) {
+    const bool picks = d > 0 && F > 0 && !(r.dedup && d <= F);
+    unsigned long long eb = (unsigned long long)(kBusRead * entry_k(d, F, r.dedup));
+    if (int_bias && picks) {
+      eb += 4ull * (unsigned long long)d;
       uint32_t* bits = d > kHubSeg ? hubbits : fbits;
       const uint32_t bit = 1u << (v & 31);
       if (!(bits[v >> 5] & bit)) atomicOr(&bits[v >> 5], bit);
     }
+    atomicAdd(&lbytes[lo], eb);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < P; i += blockDim.x) {
+    if (lcnt[i]) atomicAdd(&cnt[i], lcnt[i]);
+    if (lbytes[i]) atomicAdd(&pbytes[i], lbytes[i]);
   }
 }
 
@@ -262,6 +287,7 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
   o.sched = CSAW_SCHED_HEURISTIC;
   if (opts) o = *opts;
   if (o.mode != CSAW_MODE_PLAIN && o.mode != CSAW_MODE_BATCHED) return CSAW_ERR_INVALID_ARG;
+  if (o.slice_policy < CSAW_SLICE_AUTO || o.slice_policy > CSAW_SLICE_DIRECT) return CSAW_ERR_INVALID_ARG;
   if (o.sched < CSAW_SCHED_HEURISTIC || o.sched > CSAW_SCHED_RANDOM) return CSAW_ERR_INVALID_ARG;
   if (o.sched == CSAW_SCHED_FORCED && (o.variant < 0 || o.variant >= CSAW_NUM_VARIANTS)) return CSAW_ERR_INVALID_ARG;
   if (o.mode == CSAW_MODE_BATCHED && o.sched == CSAW_SCHED_MLP && !o.mlp) {
@@ -322,6 +348,7 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
   std::vector<int64_t> h_ho;
   double h2d_bytes = 0, h2d_ms = 0, expand_ms = 0;
   int64_t n_chunks_all = 0, var_chunks[CSAW_NUM_VARIANTS] = {};
+  int32_t direct_parts = 0, reused_slices = 0;
   auto body = [&]() -> int {
     CSAW_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     CSAW_HIP(hipStreamCreateWithFlags(&xs, hipStreamNonBlocking));
@@ -342,6 +369,20 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
     int32_t* slice[2];
     CSAW_TRY(B.alloc(&slice[0], slice_max));
     CSAW_TRY(B.alloc(&slice[1], slice_max));
+    int held[2] = {-1, -1};  // partition whose slice each buffer holds
+    int64_t last_use[2] = {0, 0}, use_tick = 0;
+    // the pinned host adjacency as the device addresses it, for partitions
+    // read in place
+    int32_t* adj_bus = nullptr;
+    if (o.slice_policy != CSAW_SLICE_COPY && E > 0 &&
+        hipHostGetDevicePointer((void**)&adj_bus, hg->adj, 0) != hipSuccess) {
+      adj_bus = nullptr;
+      (void)hipGetLastError();
+    }
+    if (o.slice_policy == CSAW_SLICE_DIRECT && E > 0 && !adj_bus) {
+      set_last_error("the host adjacency cannot be mapped for in-place reads");
+      return CSAW_ERR_UNSUPPORTED;
+    }
     // integer biases: the partition's frontier CDFs at slice positions, the
     // marks, and the hub segment lists
     uint64_t* pool = nullptr;
@@ -415,6 +456,8 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
     CSAW_TRY(B.alloc(&d_pid, mx));
     CSAW_TRY(B.alloc(&d_list, mx));
     CSAW_TRY(B.alloc(&d_cnt, 3 * P));
+    unsigned long long* d_pbytes;
+    CSAW_TRY(B.alloc(&d_pbytes, P));
     d_poff = d_cnt + P;
     d_cursor = d_cnt + 2 * P;
     CSAW_TRY(B.alloc(&d_vb, P + 1));
@@ -496,14 +539,18 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
       CSAW_TRY(launch_scans(L, H, d_tot + h, nmax[h]));
       // bin the frontier by partition
       CSAW_TRY(launch_fill(L, d_cnt, 0u, 3 * P));
+      CSAW_TRY(launch_fill(L, reinterpret_cast<uint32_t*>(d_pbytes), 0u, 2 * P));
       if (int_bias) {
         CSAW_TRY(launch_fill(L, d_hubbits, 0u, (V + 31) / 32));
         CSAW_TRY(launch_fill(L, d_fbits, 0u, (V + 31) / 32));
       }
-      k_part_bin<<<grid_of(nmax[h], num_cu), 256, 0, cs>>>(G, d_req, H, d_vb, P, d_pid, d_cnt, d_fbits, d_hubbits);
+      k_part_bin<<<grid_of(nmax[h], num_cu), 256, 0, cs>>>(G, d_req, H, d_vb, P, d_pid, d_cnt, d_pbytes, d_fbits,
+                                                           d_hubbits);
       CSAW_LAUNCH_CHECK();
       std::vector<uint32_t> cnt(P), poff(P + 1, 0);
+      std::vector<unsigned long long> pbytes(P);
       CSAW_HIP(hipMemcpyAsync(cnt.data(), d_cnt, sizeof(uint32_t) * P, hipMemcpyDeviceToHost, cs));
+      CSAW_HIP(hipMemcpyAsync(pbytes.data(), d_pbytes, sizeof(unsigned long long) * P, hipMemcpyDeviceToHost, cs));
       CSAW_HIP(hipStreamSynchronize(cs));
       for (int p = 0; p < P; ++p) poff[p + 1] = poff[p] + cnt[p];
       CSAW_HIP(hipMemcpyAsync(d_poff, poff.data(), sizeof(uint32_t) * P, hipMemcpyHostToDevice, cs));
@@ -527,24 +574,65 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
           }
         }
       }
-      // partitions in order, slice copies double-buffered on the copy stream
-      auto issue_copy = [&](int p, int buf) -> int {
-        const int64_t a = hg->off[vb[p]], b = hg->off[vb[p + 1]];
-        CSAW_HIP(hipStreamWaitEvent(xs, freed[buf], 0));
-        if (b > a) CSAW_HIP(hipMemcpyAsync(slice[buf], hg->adj + a, sizeof(int32_t) * (b - a), hipMemcpyHostToDevice, xs));
-        CSAW_HIP(hipEventRecord(ready[buf], xs));
-        h2d_bytes += (double)(b - a) * sizeof(int32_t);
+      // Partitions with frontier entries, ascending on even hops and
+      // descending on odd ones, so the two slices left on the device by one
+      // hop are the first two the next hop needs.  Each is either read in
+      // place over the bus (few bytes of it needed) or copied on the copy
+      // stream into one of two buffers, one partition ahead (double buffering).
+      std::vector<int> todo;
+      for (int q = 0; q < P; ++q) {
+        const int p = (h & 1) ? P - 1 - q : q;
+        if (cnt[p] > 0) todo.push_back(p);
+      }
+      std::vector<char> direct(P, 0);
+      for (int p = 0; p < P; ++p) {
+        const double slice_b = (double)(hg->off[vb[p + 1]] - hg->off[vb[p]]) * sizeof(int32_t);
+        const bool held_p = held[0] == p || held[1] == p;
+        if (o.slice_policy == CSAW_SLICE_DIRECT) direct[p] = 1;
+        else if (o.slice_policy == CSAW_SLICE_AUTO && adj_bus && !held_p)
+          direct[p] = (double)pbytes[p] * kDirectFactor < slice_b;
+      }
+      // *out = a buffer, other than avoid, holding partition p (the copy is
+      // issued when neither buffer holds it)
+      auto stage = [&](int p, int avoid, int* out) -> int {
+        if ((held[0] == p && avoid != 0) || (held[1] == p && avoid != 1)) {
+          *out = held[0] == p && avoid != 0 ? 0 : 1;
+          reused_slices += 1;
+          return CSAW_OK;
+        }
+        const int b = avoid >= 0 ? 1 - avoid
+                                 : (held[0] < 0 ? 0 : (held[1] < 0 ? 1 : (last_use[0] <= last_use[1] ? 0 : 1)));
+        const int64_t a = hg->off[vb[p]], e = hg->off[vb[p + 1]];
+        CSAW_HIP(hipStreamWaitEvent(xs, freed[b], 0));
+        if (e > a) CSAW_HIP(hipMemcpyAsync(slice[b], hg->adj + a, sizeof(int32_t) * (e - a), hipMemcpyHostToDevice, xs));
+        CSAW_HIP(hipEventRecord(ready[b], xs));
+        h2d_bytes += (double)(e - a) * sizeof(int32_t);
+        held[b] = p;
+        *out = b;
         return CSAW_OK;
       };
-      std::vector<int> todo;
-      for (int p = 0; p < P; ++p)
-        if (cnt[p] > 0) todo.push_back(p);
       // the freed events start signalled
       for (int i = 0; i < 2; ++i) CSAW_HIP(hipEventRecord(freed[i], cs));
-      if (!todo.empty()) CSAW_TRY(issue_copy(todo[0], 0));
+      std::vector<int> bufof(todo.size(), -1);
+      auto next_copy = [&](size_t from) -> size_t {
+        while (from < todo.size() && direct[todo[from]]) ++from;
+        return from;
+      };
+      size_t staged = next_copy(0);
+      if (staged < todo.size()) CSAW_TRY(stage(todo[staged], -1, &bufof[staged]));
       for (size_t t = 0; t < todo.size(); ++t) {
-        const int p = todo[t], buf = (int)(t & 1);
-        if (t + 1 < todo.size()) CSAW_TRY(issue_copy(todo[t + 1], (int)((t + 1) & 1)));
+        const int p = todo[t];
+        const int buf = direct[p] ? -1 : bufof[t];
+        if (direct[p]) direct_parts += 1;
+        if (buf >= 0) {
+          // stage the next copied partition into the other buffer
+          const size_t t2 = next_copy(t + 1);
+          if (t2 < todo.size() && t2 > staged) {
+            staged = t2;
+            CSAW_TRY(stage(todo[t2], buf, &bufof[t2]));
+          }
+          last_use[buf] = ++use_tick;
+        }
         hipEvent_t eb0, eb1, ex0, ex1;
         CSAW_TRY(event(&eb0));
         CSAW_TRY(event(&eb1));
@@ -574,10 +662,11 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
                                   sizeof(uint32_t) * kNumQueues, hipMemcpyDeviceToHost, cs));
         }
         CSAW_HIP(hipEventRecord(eb1, cs));
-        CSAW_HIP(hipStreamWaitEvent(cs, ready[buf], 0));
+        if (buf >= 0) CSAW_HIP(hipStreamWaitEvent(cs, ready[buf], 0));
         CSAW_HIP(hipEventRecord(ex0, cs));
         GraphDev Gp = G;
-        Gp.adj = slice[buf] - hg->off[vb[p]];  // global offsets index the slice
+        // global offsets index the slice (or the host adjacency, read in place)
+        Gp.adj = buf >= 0 ? slice[buf] - hg->off[vb[p]] : adj_bus;
         csaw_graph_view hvp = hv;
         hvp.adj_list = Gp.adj;
         Gp.n_hubs = 0;
@@ -637,7 +726,7 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
           CSAW_TRY(launch_expand_list(L, kind, Gp, hvp, d_req, d_gs2req, H, d_list + poff[p], (int64_t)cnt[p]));
         }
         CSAW_HIP(hipEventRecord(ex1, cs));
-        CSAW_HIP(hipEventRecord(freed[buf], cs));
+        if (buf >= 0) CSAW_HIP(hipEventRecord(freed[buf], cs));
         if (!seg_hub.empty()) CSAW_HIP(hipStreamSynchronize(cs));  // hub list buffers are reused
       }
     }
@@ -715,6 +804,8 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
     stats->peak_device_bytes = B.peak;
     stats->slice_bytes = slice_max * (int64_t)sizeof(int32_t);
     stats->expand_ms = expand_ms;
+    stats->direct_partitions = direct_parts;
+    stats->reused_slices = reused_slices;
     stats->chunks = n_chunks_all;
     for (int v = 0; v < CSAW_NUM_VARIANTS; ++v) stats->variant_chunks[v] = var_chunks[v];
   }

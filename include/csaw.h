@@ -290,7 +290,18 @@ typedef struct csaw_stream_stats {
                              /*  expansion (copy waits excluded)            */
   int64_t chunks;            /* chunks scheduled (batched mode)             */
   int64_t variant_chunks[CSAW_NUM_VARIANTS]; /* chunks run per variant      */
+  int32_t direct_partitions; /* (hop, partition) pairs read in place        */
+  int32_t reused_slices;     /* slices a hop found left on the device       */
 } csaw_stream_stats;
+
+/* Where a partition's adjacency comes from in a hop. */
+typedef enum csaw_slice_policy {
+  CSAW_SLICE_AUTO = 0,   /* copy the slice, or read the pinned host lists in */
+                         /*  place when the partition's frontier needs less */
+                         /*  than half the slice's bytes                    */
+  CSAW_SLICE_COPY = 1,   /* always copy slices (double-buffered)            */
+  CSAW_SLICE_DIRECT = 2  /* always read in place                            */
+} csaw_slice_policy;
 
 /* How each partition's frontier entries are expanded. */
 typedef struct csaw_stream_opts {
@@ -303,6 +314,7 @@ typedef struct csaw_stream_opts {
   uint64_t explore_key; /* key of CSAW_SCHED_RANDOM                          */
   const struct csaw_mlp *mlp;       /* cost model for CSAW_SCHED_MLP or NULL */
   const struct csaw_tuning *tuning; /* launch shapes or NULL (defaults)      */
+  int32_t slice_policy; /* csaw_slice_policy                                 */
 } csaw_stream_opts;
 
 /* Samples one request (uniform, degree or user bias) streaming vertex-range

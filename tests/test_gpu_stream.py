@@ -99,6 +99,43 @@ def test_stream_hub_schedules(C, hub_deg):
             same(got, want, f"hub {hub_deg} bias {bias} {name}")
 
 
+@pytest.mark.parametrize("bias", [0, 1, 3])
+def test_stream_slice_policies(C, graphs, bias):
+    # copied slices, in-place reads of the pinned host lists, and the
+    # automatic per-(hop, partition) choice give the same edges
+    g, hg, _ = graphs
+    seeds = g.seeds(1200, request=20 + bias)
+    want = C.sample(g, seeds, [15, 10, 5], bias=bias, request=4)
+    parts = 7
+    for pol in (C.SLICE_COPY, C.SLICE_DIRECT, C.SLICE_AUTO):
+        for name, kw in (("plain", dict(mode=C.MODE_PLAIN)), ("heuristic", dict()),
+                         ("forced0", dict(sched=C.SCHED_FORCED, variant=0)),
+                         ("forced3", dict(sched=C.SCHED_FORCED, variant=3))):
+            got = C.stream_sample(hg, seeds, [15, 10, 5], bias=bias, request=4, partitions=parts,
+                                  budget_bytes=1 << 30, slice_policy=pol, **kw)
+            same(got, want, f"bias {bias} policy {pol} {name}")
+            st = got["stats"]
+            if pol == C.SLICE_COPY:
+                assert st.direct_partitions == 0
+                # hops alternate direction: the next hop starts on the two slices left
+                assert st.reused_slices >= 2
+            if pol == C.SLICE_DIRECT:
+                assert st.h2d_bytes == 0 and st.reused_slices == 0
+                assert st.direct_partitions >= parts
+
+
+def test_stream_auto_reads_sparse_hop_in_place(C):
+    # 4 seeds on a 2^15-vertex graph: the first hop touches a few lists of
+    # each partition, so no slice is copied for it
+    g = C.Graph.rmat(15, 16)
+    off, adj = g.csr()
+    hg = C.HostGraph.from_csr(off, adj)
+    seeds = g.seeds(4, request=1)
+    got = C.stream_sample(hg, seeds, [3], bias=C.BIAS_DEGREE, partitions=8, budget_bytes=1 << 28)
+    same(got, C.sample(g, seeds, [3], bias=C.BIAS_DEGREE))
+    assert got["stats"].h2d_bytes == 0 and got["stats"].direct_partitions >= 1
+
+
 def test_stream_matches_oracle(C, O, graphs):
     g, hg, og = graphs
     seeds = g.seeds(500, request=3)
@@ -124,7 +161,7 @@ def test_stream_budget_too_small(C, graphs):
 def test_stream_rejects_bad_opts(C, graphs):
     g, hg, _ = graphs
     for kw in (dict(mode=5), dict(sched=9), dict(sched=C.SCHED_FORCED, variant=C.NUM_VARIANTS),
-               dict(sched=C.SCHED_MLP, mlp=None)):
+               dict(sched=C.SCHED_MLP, mlp=None), dict(slice_policy=7)):
         with pytest.raises(C.CsawError) as ei:
             C.stream_sample(hg, g.seeds(10), [5], budget_bytes=1 << 28, **kw)
         assert ei.value.status == 1
