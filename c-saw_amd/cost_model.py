@@ -60,6 +60,35 @@ def collect(C, g, specs, cdf_cache=True, wg_per_cu=None):
     return {"keys": keys, "X": X, "T": T, "M": M, "heur": heur}
 
 
+STREAM_MODEL = os.path.join(MODEL_DIR, "cost_model_gfx950_stream.json")
+
+
+def collect_stream(C, hg, seeds, fanouts, bias, partitions=16, budget_bytes=4 << 30, wg_per_cu=None,
+                   log_cap=1 << 20):
+    """collect() for the streaming sampler: one streamed run per schedule
+    (heuristic, then each forced variant) with per-chunk logging, joined by
+    chunk key (hop, partition, chunk)."""
+    wg = dict(WG_PER_CU)
+    if wg_per_cu:
+        wg.update(wg_per_cu)
+    kw = dict(bias=bias, partitions=partitions, budget_bytes=budget_bytes, log_cap=log_cap)
+    X0, hv, _, hk = C.stream_sample(hg, seeds, fanouts, sched=C.SCHED_HEURISTIC, **kw)["log"]
+    order = np.argsort(hk)
+    keys = hk[order]
+    X = X0[order]
+    heur = hv[order]
+    T = np.zeros((len(keys), C.NUM_VARIANTS), np.float32)
+    M = np.zeros((len(keys), C.NUM_VARIANTS), np.float32)
+    for v in range(C.NUM_VARIANTS):
+        _, var, lt, k = C.stream_sample(hg, seeds, fanouts, sched=C.SCHED_FORCED, variant=v, **kw)["log"]
+        pos = np.searchsorted(keys, k)
+        ok = (pos < len(keys)) & (var == v)
+        ok[ok] &= keys[pos[ok]] == k[ok]
+        T[pos[ok], v] = lt[ok] - np.log2(wg[v])
+        M[pos[ok], v] = 1.0
+    return {"keys": keys, "X": X, "T": T, "M": M, "heur": heur}
+
+
 def stream_specs(C, g, n, mix_key, lo=64, hi=65536):
     """A request mix the streaming sampler accepts: the C4 mix with FP32
     weight bias replaced by degree bias."""

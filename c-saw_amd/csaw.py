@@ -98,6 +98,7 @@ _sig("csaw_plan_stats", [P, P])
 _sig("csaw_plan_request_info", [P, i32, P])
 _sig("csaw_plan_copy", [P, i32, P, P, P, P])
 _sig("csaw_plan_kernel_times", [P, ctypes.c_char_p, ctypes.c_size_t, P, P, P, i32, P])
+_sig("csaw_plan_copy_device", [P, i32, P, P, P, P, P, ctypes.c_int])
 _sig("csaw_plan_digest", [P, i32, P])
 _sig("csaw_plan_reset_times", [P])
 _sig("csaw_plan_destroy", [P])
@@ -442,7 +443,8 @@ class StreamStats(ctypes.Structure):
 
 class StreamOpts(ctypes.Structure):
     _fields_ = [("mode", i32), ("sched", i32), ("variant", i32), ("explore_key", u64), ("mlp", P), ("tuning", P),
-                ("slice_policy", i32)]
+                ("slice_policy", i32), ("log_chunks", i32), ("log_cap", i64), ("log_features", P),
+                ("log_variant", P), ("log_log2_ticks", P), ("log_keys", P), ("log_n", P)]
 
 
 _sig("csaw_host_graph_create_rmat", [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -503,14 +505,25 @@ class HostGraph:
 
 def stream_sample(hg, seeds, fanouts, bias=BIAS_DEGREE, dedup=1, key=42, request=0, budget_bytes=4 << 30,
                   partitions=16, mode=MODE_BATCHED, sched=SCHED_HEURISTIC, variant=0, mlp=None, explore_key=0,
-                  tuning=None, slice_policy=0):
+                  tuning=None, slice_policy=0, log_cap=0):
     """Streams one request; mode / sched / variant / mlp choose how each
-    partition's entries are expanded (the edge list is the same for all)."""
+    partition's entries are expanded (the edge list is the same for all).
+    log_cap > 0 also returns the per-chunk cost-model rows as "log":
+    (features [n,16], variant, log2 ticks, keys)."""
     reqs, keep = make_requests([{"seeds": seeds, "fanouts": fanouts, "bias": bias, "dedup": dedup, "key": key,
                                  "request": request}])
     e = P()
     st = StreamStats()
+    logs = None
+    log_n = i64()
+    if log_cap > 0:
+        logs = (np.zeros((log_cap, 16), np.float32), np.zeros(log_cap, np.int32), np.zeros(log_cap, np.float32),
+                np.zeros(log_cap, np.uint64))
     o = StreamOpts(mode=mode, sched=sched, variant=variant, explore_key=explore_key, slice_policy=slice_policy,
+                   log_chunks=1 if logs else 0, log_cap=log_cap,
+                   log_features=_p(logs[0]) if logs else None, log_variant=_p(logs[1]) if logs else None,
+                   log_log2_ticks=_p(logs[2]) if logs else None, log_keys=_p(logs[3]) if logs else None,
+                   log_n=ctypes.cast(ctypes.byref(log_n), P) if logs else None,
                    mlp=mlp.h if mlp is not None else None,
                    tuning=ctypes.cast(ctypes.byref(tuning), P) if tuning is not None else None)
     _check(_L.csaw_stream_sample_opts(hg.h, reqs, budget_bytes, partitions, ctypes.byref(o), ctypes.byref(e),
@@ -526,8 +539,14 @@ def stream_sample(hg, seeds, fanouts, bias=BIAS_DEGREE, dedup=1, key=42, request
     finally:
         _L.csaw_edges_free(e)
     k = n.value
-    return {"src": src[:k], "dst": dst[:k], "smp": smp[:k],
-            "hop_off": ho[:d.value * (s.value + 1)].reshape(d.value, s.value + 1), "stats": st}
+    out = {"src": src[:k], "dst": dst[:k], "smp": smp[:k],
+           "hop_off": ho[:d.value * (s.value + 1)].reshape(d.value, s.value + 1), "stats": st}
+    if logs:
+        m = log_n.value
+        if m > log_cap:
+            raise CsawError(6, f"stream_sample: {m} chunk rows exceed log_cap {log_cap}")
+        out["log"] = tuple(a[:m] for a in logs)
+    return out
 
 
 def mdrw(g, seeds, samples, frontier_size, depth, key=42, request=0):

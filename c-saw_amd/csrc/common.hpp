@@ -133,6 +133,14 @@ __host__ __device__ __forceinline__ uint32_t chunk_max_for(int64_t n, int32_t wa
   return c;
 }
 
+// Sub-wave group size for a largest fanout: the fanout itself (at least
+// kGroupMin), so that e.g. fanout 5 runs 12 entries per wave round instead of
+// 8; above 32 one group is the whole wave.
+constexpr int kGroupMin = 4;
+__host__ __device__ __forceinline__ int group_size_for(int max_f) {
+  return max_f <= kGroupMin ? kGroupMin : (max_f <= 32 ? max_f : kWave);
+}
+
 __host__ __device__ __forceinline__ int ceil_log2_u32(uint32_t d) {
   // smallest c with 2^c >= d (d >= 1)
 #if defined(__HIP_DEVICE_COMPILE__)

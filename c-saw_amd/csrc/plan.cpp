@@ -286,7 +286,7 @@ static int run_unit(csaw_plan* p, Unit& u, hipStream_t st, TimingHook* th) {
       int fmax = 0;
       for (int ri : u.reqs)
         if (p->reqs[ri].depth > h) fmax = std::max(fmax, p->reqs[ri].fanouts[h]);
-      const int gs = fmax <= 8 ? 8 : (fmax <= 16 ? 16 : (fmax <= 32 ? 32 : 64));
+      const int gs = group_size_for(fmax);
       if (th) th->begin(kKernPlain, st);
       CSAW_TRY(launch_expand_plain(L, kind, G, hv, u.d_req, u.d_gs2req, H, u.nmax[h], gs));
       if (th) th->end(kKernPlain, st);
@@ -522,6 +522,54 @@ extern "C" int csaw_plan_copy(const csaw_plan* p, int32_t req, int32_t* src, int
     }
     at += n;
   }
+  return CSAW_OK;
+}
+
+namespace csaw {
+__global__ void k_sub_i32(int32_t* __restrict__ x, int64_t n, int32_t sub) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] -= sub;
+}
+__global__ void k_add_i64(const int64_t* __restrict__ in, int64_t n, int64_t add, int64_t* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = in[i] + add;
+}
+}  // namespace csaw
+
+// csaw_plan_copy into device buffers, on the device (for callers that keep
+// the edge lists on the GPU, e.g. as framework tensors).
+extern "C" int csaw_plan_copy_device(const csaw_plan* p, int32_t req, int32_t* src, int32_t* dst, int32_t* sample,
+                                     int64_t* hop_offsets, void* stream, int sync) {
+  if (!p) return CSAW_ERR_INVALID_ARG;
+  std::vector<int64_t> lo, hi;
+  CSAW_TRY(request_bounds(p, req, lo, hi));
+  const auto [ui, k] = p->req_unit[req];
+  const Unit& u = p->units[ui];
+  const csaw_request& r = p->reqs[req];
+  hipStream_t st = stream ? (hipStream_t)stream : p->stream;
+  auto grid = [&](int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096)); };
+  int64_t at = 0;
+  for (int h = 0; h < r.depth; ++h) {
+    const int64_t n = hi[h] - lo[h];
+    if (n > 0) {
+      if (src) CSAW_HIP(hipMemcpyAsync(src + at, u.d_src[h] + lo[h], sizeof(int32_t) * n, hipMemcpyDeviceToDevice, st));
+      if (dst) CSAW_HIP(hipMemcpyAsync(dst + at, u.d_dst[h] + lo[h], sizeof(int32_t) * n, hipMemcpyDeviceToDevice, st));
+      if (sample) {
+        CSAW_HIP(hipMemcpyAsync(sample + at, u.d_smp[h] + lo[h], sizeof(int32_t) * n, hipMemcpyDeviceToDevice, st));
+        if (u.gbase[k]) {
+          k_sub_i32<<<grid(n), 256, 0, st>>>(sample + at, n, (int32_t)u.gbase[k]);
+          CSAW_LAUNCH_CHECK();
+        }
+      }
+    }
+    if (hop_offsets) {
+      k_add_i64<<<grid(r.samples + 1), 256, 0, st>>>(u.d_eo[h] + u.gbase[k], r.samples + 1, at - lo[h],
+                                                     hop_offsets + (int64_t)h * (r.samples + 1));
+      CSAW_LAUNCH_CHECK();
+    }
+    at += n;
+  }
+  if (sync) CSAW_HIP(hipStreamSynchronize(st));
   return CSAW_OK;
 }
 

@@ -274,7 +274,7 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, int64_t n) {
 }
 
 // ----------------------------------------------------------- expand plain
-// gs: group size for the hop's fanout (8, 16, 32 or 64; 0 = the whole-wave
+// gs: group size for the hop's fanout (group_size_for; 0 = the whole-wave
 // path), used for the kinds that need no per-entry CDF build (uniform, FP32).
 template <int KIND>
 __global__ __launch_bounds__(256) void k_expand_plain(GraphDev G, csaw_graph_view hv, const ReqDev* __restrict__ R,

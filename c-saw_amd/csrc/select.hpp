@@ -496,7 +496,8 @@ __device__ __forceinline__ int32_t gpick_small(const GEntry& en, const uint32_t*
 }
 
 // lc: the group's LDS slot of kGroupSmallCap words (null: no small path)
-// gs: group size (8, 16, 32 or 64), a run-time value so one build serves all
+// gs: group size (4 .. 32, or 64), a run-time value so one build serves all;
+// the group is lanes gbase .. gbase + gs - 1 (gmask)
 template <int KIND, int KEYS>
 __device__ int32_t group_select(const GraphDev& G, const GEntry& en, int gs, int gl, int gbase, uint64_t gmask,
                                 uint32_t* lc) {
@@ -538,14 +539,11 @@ __device__ int32_t group_select(const GraphDev& G, const GEntry& en, int gs, int
   const bool use_mask = d <= 128;
   uint64_t mlo = 0, mhi = 0;
   if (use_mask) {
-    uint64_t blo = (gl < j0 && cand < 64) ? (1ull << cand) : 0ull;
-    uint64_t bhi = (gl < j0 && cand >= 64) ? (1ull << (cand - 64)) : 0ull;
-    for (int off = 1; off < gs; off <<= 1) {  // OR over the group (lane ^ off stays in it)
-      blo |= shfl_abs_u64(blo, lane_id() ^ off);
-      bhi |= shfl_abs_u64(bhi, lane_id() ^ off);
+    for (int m = 0; m < j0; ++m) {  // the resolved slots 0 .. j0-1
+      const int32_t cm = shfl_abs(cand, gbase + m);
+      if (cm < 64) mlo |= 1ull << cm;
+      else mhi |= 1ull << (cm - 64);
     }
-    mlo = blo;
-    mhi = bhi;
   }
   auto in_mask = [&](int32_t ci) -> bool { return ((ci < 64 ? mlo >> ci : mhi >> (ci - 64)) & 1ull) != 0ull; };
   for (int jj = j0; jj < F; ++jj) {
@@ -589,26 +587,23 @@ __device__ int32_t group_select(const GraphDev& G, const GEntry& en, int gs, int
 }
 
 // Expands `count` (<= 64) entries, all with F <= gs, in groups of gs lanes
-// (gs = 8, 16, 32 or 64).  Needs no per-entry CDF build: uniform, FP32, or
-// integer biases with the CDF cache.  lds: the wave's kGroupLdsWords words for
-// small-degree CDFs (null: searches only).
-constexpr int kGroupLdsWords = (kWave / 8) * kGroupSmallCap;
-
-__device__ __forceinline__ int group_size_for(int max_f) {
-  return max_f <= 8 ? 8 : (max_f <= 16 ? 16 : (max_f <= 32 ? 32 : kWave));
-}
+// (4 <= gs <= 32, or 64): 64 / gs groups, the remaining 64 mod gs lanes idle.
+// Needs no per-entry CDF build: uniform, FP32, or integer biases with the CDF
+// cache.  lds: the wave's kGroupLdsWords words for small-degree CDFs (null:
+// searches only).
+constexpr int kGroupLdsWords = (kWave / kGroupMin) * kGroupSmallCap;
 
 template <int KIND, int KEYS = kKeysAny>
 __device__ void group_expand_batch(const GraphDev& G, const csaw_graph_view& hv, const ReqDev* __restrict__ R,
                                    const int32_t* __restrict__ gs2req, const HopDev& H,
                                    const int32_t* __restrict__ order, int64_t first, int count, int gs,
                                    uint32_t* lds) {
-  const int lg = __builtin_ctz((unsigned)gs);
-  const int NG = kWave >> lg;
+  const int NG = kWave / gs;
   const int lane = lane_id();
-  const int gl = lane & (gs - 1), gbase = lane & ~(gs - 1);
-  uint32_t* lc = lds ? lds + (gbase >> lg) * kGroupSmallCap : nullptr;
-  const uint64_t gmask = (gs >= kWave ? ~0ull : ((1ull << gs) - 1ull)) << gbase;
+  const int gi = lane / gs;  // group; gi == NG: the idle remainder lanes
+  const int gl = lane - gi * gs, gbase = gi * gs;
+  uint32_t* lc = lds && gi < NG ? lds + gi * kGroupSmallCap : nullptr;
+  const uint64_t gmask = gs >= kWave ? ~0ull : (((1ull << gs) - 1ull) << gbase);
   GEntry mine;
   mine.d = 0;
   mine.F = 0;
@@ -637,7 +632,7 @@ __device__ void group_expand_batch(const GraphDev& G, const csaw_graph_view& hv,
   int32_t pv = 0, pg = 0, pnb = 0;
   int64_t pp = 0;
   for (int e0 = 0; e0 < count; e0 += NG) {
-    const int e = e0 + (lane >> lg);
+    const int e = gi < NG ? e0 + gi : count;
     const GEntry en = shfl_gentry(mine, e < count ? e : 0);
     int32_t idx = -1;
     if (e < count && en.d > 0 && en.F > 0) {

@@ -23,6 +23,7 @@
 // expands each listed entry.
 // Every device allocation is charged against budget_bytes.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -349,6 +350,7 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
   double h2d_bytes = 0, h2d_ms = 0, expand_ms = 0;
   int64_t n_chunks_all = 0, var_chunks[CSAW_NUM_VARIANTS] = {};
   int32_t direct_parts = 0, reused_slices = 0;
+  int64_t log_rows = 0;
   auto body = [&]() -> int {
     CSAW_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     CSAW_HIP(hipStreamCreateWithFlags(&xs, hipStreamNonBlocking));
@@ -471,6 +473,7 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
     Chunk* d_chunks = nullptr;
     uint32_t *d_queues = nullptr, *d_ctr = nullptr;
     uint16_t* d_mlp = nullptr;
+    ChunkLog* d_log = nullptr;
     uint32_t chunk_cap = 0;
     const int32_t expand_waves = L.num_cu * L.expand_per_cu * 4;
     if (batched) {
@@ -480,6 +483,7 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
       CSAW_TRY(B.alloc(&d_queues, (int64_t)kNumQueues * chunk_cap));
       CSAW_TRY(B.alloc(&d_ctr, kCtrCount));
       CSAW_TRY(launch_fill(L, d_ctr, 0u, kCtrCount));
+      if (o.log_chunks) CSAW_TRY(B.alloc(&d_log, chunk_cap));
       if (o.sched == CSAW_SCHED_MLP) {
         CSAW_TRY(B.alloc(&d_mlp, kMlpParams));
         CSAW_TRY(mlp_upload_bf16(o.mlp, d_mlp));
@@ -719,7 +723,7 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
           X.queue_cap = chunk_cap;
           X.qlen = d_ctr + kCtrQlen;
           X.qhead = d_ctr + kCtrQhead;
-          X.log = nullptr;
+          X.log = d_log;
           X.int_cache = int_bias ? 1 : 0;
           CSAW_TRY(launch_expand_chunks(L, Gp, hvp, d_req, d_gs2req, H, X, kinds_present, nullptr));
         } else {
@@ -728,6 +732,32 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
         CSAW_HIP(hipEventRecord(ex1, cs));
         if (buf >= 0) CSAW_HIP(hipEventRecord(freed[buf], cs));
         if (!seg_hub.empty()) CSAW_HIP(hipStreamSynchronize(cs));  // hub list buffers are reused
+        if (d_log) {
+          // this partition's chunks and their timings, before the next reuses them
+          CSAW_HIP(hipStreamSynchronize(cs));
+          uint32_t nch = 0;
+          CSAW_HIP(hipMemcpy(&nch, d_ctr + kCtrChunks, sizeof(uint32_t), hipMemcpyDeviceToHost));
+          nch = std::min(nch, chunk_cap);
+          std::vector<Chunk> ch(nch);
+          std::vector<ChunkLog> lg(nch);
+          if (nch) {
+            CSAW_HIP(hipMemcpy(ch.data(), d_chunks, sizeof(Chunk) * nch, hipMemcpyDeviceToHost));
+            CSAW_HIP(hipMemcpy(lg.data(), d_log, sizeof(ChunkLog) * nch, hipMemcpyDeviceToHost));
+          }
+          for (uint32_t i = 0; i < nch; ++i) {
+            const Chunk& c = ch[i];
+            if (c.bucket == kBucketCopy) continue;
+            if (log_rows < o.log_cap) {
+              if (o.log_features)
+                chunk_features(c.len, c.sum_d, c.max_d, c.sum_f, c.max_f, bucket_kind(c.bucket), c.hop, c.bucket, nch,
+                               o.log_features + log_rows * kMlpIn);
+              if (o.log_variant) o.log_variant[log_rows] = lg[i].variant;
+              if (o.log_log2_ticks) o.log_log2_ticks[log_rows] = std::log2((float)std::max<uint32_t>(lg[i].ticks, 1u));
+              if (o.log_keys) o.log_keys[log_rows] = ((uint64_t)h << 40) | ((uint64_t)p << 32) | c.key;
+            }
+            ++log_rows;
+          }
+        }
       }
     }
     CSAW_HIP(hipEventRecord(t1, cs));
@@ -795,6 +825,7 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
   if (xs) (void)hipStreamDestroy(xs);
   if (h_qlen) (void)hipHostFree(h_qlen);
   if (st != CSAW_OK) return st;
+  if (o.log_n) *o.log_n = log_rows;
   if (stats) {
     memset(stats, 0, sizeof(*stats));
     stats->edges = (int64_t)h_src.size();

@@ -243,6 +243,13 @@ int csaw_plan_stats(const csaw_plan *p, csaw_stats *stats);
 int csaw_plan_request_info(const csaw_plan *p, int32_t req, int64_t *n_edges);
 int csaw_plan_copy(const csaw_plan *p, int32_t req, int32_t *src, int32_t *dst, int32_t *sample,
                    int64_t *hop_offsets);
+/* csaw_plan_copy into device buffers (src/dst/sample: csaw_plan_request_info
+ * entries, hop_offsets: depth*(samples+1)), enqueued on stream (hipStream_t,
+ * NULL: the plan's stream); blocks until done when sync != 0.  For callers
+ * that keep the edge lists on the GPU (framework tensors); no counterpart in
+ * the reference, whose Si stayed in device memory (README.md:36). */
+int csaw_plan_copy_device(const csaw_plan *p, int32_t req, int32_t *src, int32_t *dst, int32_t *sample,
+                          int64_t *hop_offsets, void *stream, int sync);
 /* Order-sensitive 64-bit digest of request req's edge list (src, dst,
  * sample, position), computed on the device: equal edge lists give equal
  * digests whichever plan mode or variant produced them. */
@@ -315,6 +322,16 @@ typedef struct csaw_stream_opts {
   const struct csaw_mlp *mlp;       /* cost model for CSAW_SCHED_MLP or NULL */
   const struct csaw_tuning *tuning; /* launch shapes or NULL (defaults)      */
   int32_t slice_policy; /* csaw_slice_policy                                 */
+  /* Per-chunk cost-model data (batched mode; each partition's chunks are
+   * read back after it expands, so logging runs slower).  Rows as in
+   * csaw_plan_chunk_log; keys are hop << 40 | partition << 32 | chunk key. */
+  int32_t log_chunks;
+  int64_t log_cap;            /* rows the buffers below hold                */
+  float *log_features;        /* [log_cap][16]                              */
+  int32_t *log_variant;
+  float *log_log2_ticks;
+  uint64_t *log_keys;
+  int64_t *log_n;             /* rows produced (may exceed log_cap)         */
 } csaw_stream_opts;
 
 /* Samples one request (uniform, degree or user bias) streaming vertex-range

@@ -1,9 +1,13 @@
 """Streaming sampler: partitions streamed through a budgeted device must give
 exactly the in-memory (and oracle) edge lists."""
+import os
+import sys
+
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "c-saw_amd"))
 KEYS = ("src", "dst", "smp", "hop_off")
 
 
@@ -172,3 +176,34 @@ def test_stream_rejects_weights(C, graphs):
     with pytest.raises(C.CsawError) as ei:
         C.stream_sample(hg, g.seeds(10), [5], bias=C.BIAS_WEIGHT)
     assert ei.value.status == 8
+
+
+def test_stream_chunk_log(C, graphs):
+    g, hg, _ = graphs
+    seeds = g.seeds(800, request=5)
+    for v in (0, 3):
+        r = C.stream_sample(hg, seeds, [15, 10], bias=C.BIAS_DEGREE, partitions=4, budget_bytes=1 << 30,
+                            sched=C.SCHED_FORCED, variant=v, log_cap=1 << 16)
+        X, var, lt, keys = r["log"]
+        st = r["stats"]
+        assert 0 < len(keys) <= st.chunks
+        assert X.shape == (len(keys), 16) and np.all(np.isfinite(X)) and np.all(lt >= 0)
+        assert len(np.unique(keys)) == len(keys)
+        assert set((keys >> 40).tolist()) <= {0, 1} and set(((keys >> 32) & 0xff).tolist()) <= {0, 1, 2, 3}
+        # degree bias with every drawing vertex's CDF built: both variants valid for every chunk
+        assert np.all(var == v)
+
+
+def test_stream_learned_scheduler_beats_heuristic(C):
+    import cost_model as CM
+    hg = C.HostGraph.rmat(18, 16)
+    fan = [15, 10, 5]
+    train = CM.collect_stream(C, hg, C.seeds_v(hg.V, 4096, key=7), fan, C.BIAS_DEGREE, partitions=8,
+                              budget_bytes=1 << 30)
+    test = CM.collect_stream(C, hg, C.seeds_v(hg.V, 4096, key=11), fan, C.BIAS_DEGREE, partitions=8,
+                             budget_bytes=1 << 30)
+    mlp, loss = CM.train(C, train)
+    r = CM.regret(test, mlp)
+    print("streaming held-out regret", r)
+    assert r["chunks"] > 100
+    assert r["mlp"] < r["heuristic"], r
