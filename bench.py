@@ -302,6 +302,8 @@ def main():
     ap.add_argument("--workload", choices=["c4", "c1", "c2", "c3", "uniform", "c5"], default="c4",
                     help="c4 (default) is the headline; the others measure the remaining configs")
     ap.add_argument("--no-graph", action="store_true", help="--only batched: launch eagerly (profilers)")
+    ap.add_argument("--serial-expand", action="store_true",
+                    help="run each hop's variant kernels one after another on one stream")
     ap.add_argument("--only", choices=["all", "batched", "plain", "copy", "gather"], default="all",
                     help="profiling modes: run one executor only (or a 256 MiB copy for counter calibration)")
     a = ap.parse_args()
@@ -371,7 +373,7 @@ def main():
             p = C.Plan(g, specs, mode=C.MODE_PLAIN)
         else:
             p = C.Plan(g, specs, mode=C.MODE_BATCHED, sched=C.SCHED_MLP, mlp=mlp, use_graph=not a.no_graph,
-                       tuning=tuning, cdf_cache=True)
+                       tuning=tuning, cdf_cache=True, serial_expand=a.serial_expand)
         t = run_timed(C, p, a.steps, a.warmup)
         e = sum(p.n_edges(i) for i in range(len(specs)))
         print(json.dumps({"only": a.only, "ms_per_step": t * 1e3, "edges": e, "value": e / t}))
@@ -379,7 +381,7 @@ def main():
 
     plain = C.Plan(g, specs, mode=C.MODE_PLAIN)
     batched = C.Plan(g, specs, mode=C.MODE_BATCHED, sched=C.SCHED_MLP, mlp=mlp, use_graph=True, tuning=tuning,
-                     cdf_cache=True)
+                     cdf_cache=True, serial_expand=a.serial_expand)
     t_plain = run_timed(C, plain, max(1, min(a.steps, 3)), 1)
     t_bat = run_timed(C, batched, a.steps, a.warmup)
     edges = sum(batched.n_edges(i) for i in range(len(specs)))

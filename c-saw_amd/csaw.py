@@ -56,7 +56,8 @@ class Tuning(ctypes.Structure):
 
 class PlanOpts(ctypes.Structure):
     _fields_ = [("mode", i32), ("sched", i32), ("variant", i32), ("use_graph", i32), ("timing", i32),
-                ("log_chunks", i32), ("explore_key", u64), ("mlp", P), ("tuning", P), ("cdf_cache", i32)]
+                ("log_chunks", i32), ("explore_key", u64), ("mlp", P), ("tuning", P), ("cdf_cache", i32),
+                ("serial_expand", i32)]
 
 
 if not os.path.exists(LIB_PATH):
@@ -286,7 +287,7 @@ def request_mix(n=256, lo=64, hi=65536, mix_key=42):
 
 class Plan:
     def __init__(self, g, specs, mode=MODE_BATCHED, sched=SCHED_HEURISTIC, variant=0, use_graph=False, timing=False,
-                 log_chunks=False, explore_key=0, mlp=None, tuning=None, cdf_cache=False):
+                 log_chunks=False, explore_key=0, mlp=None, tuning=None, cdf_cache=False, serial_expand=False):
         self.g = g
         self.specs = specs
         self.reqs, self._keep = make_requests(specs)
@@ -295,7 +296,7 @@ class Plan:
                         timing=1 if timing else 0, log_chunks=1 if log_chunks else 0, explore_key=explore_key,
                         mlp=mlp.h if mlp is not None else None,
                         tuning=ctypes.cast(ctypes.byref(tuning), P) if tuning is not None else None,
-                        cdf_cache=1 if cdf_cache else 0)
+                        cdf_cache=1 if cdf_cache else 0, serial_expand=1 if serial_expand else 0)
         self.h = P()
         _check(_L.csaw_plan_create(g.h, self.reqs, len(specs), ctypes.byref(opts), ctypes.byref(self.h)),
                "plan_create")

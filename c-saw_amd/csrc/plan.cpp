@@ -107,6 +107,11 @@ static void plan_free(csaw_plan* p) {
   (void)hipFree(p->bytes_acc);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
+  for (int i = 0; i < csaw_plan::kSide; ++i) {
+    if (p->side[i]) (void)hipStreamDestroy(p->side[i]);
+    if (p->join[i]) (void)hipEventDestroy(p->join[i]);
+  }
+  if (p->fork) (void)hipEventDestroy(p->fork);
   delete p;
 }
 
@@ -272,7 +277,24 @@ static int run_unit(csaw_plan* p, Unit& u, hipStream_t st, TimingHook* th) {
       E.qhead = p->counters + kCtrQhead;
       E.log = p->opts.log_chunks ? p->log : nullptr;
       E.int_cache = p->opts.cdf_cache != 0;
-      CSAW_TRY(launch_expand_chunks(L, G, hv, u.d_req, u.d_gs2req, H, E, u.kinds, th));
+      if (!th && p->side[0]) {
+        // the hop's expansion kernels (one per variant queue) run side by
+        // side on four streams, so one kernel's last chunks overlap the
+        // others' work; all join before the next hop
+        hipStream_t ss[1 + csaw_plan::kSide] = {st};
+        CSAW_HIP(hipEventRecord(p->fork, st));
+        for (int i = 0; i < csaw_plan::kSide; ++i) {
+          CSAW_HIP(hipStreamWaitEvent(p->side[i], p->fork, 0));
+          ss[1 + i] = p->side[i];
+        }
+        CSAW_TRY(launch_expand_chunks(L, G, hv, u.d_req, u.d_gs2req, H, E, u.kinds, nullptr, ss, 1 + csaw_plan::kSide));
+        for (int i = 0; i < csaw_plan::kSide; ++i) {
+          CSAW_HIP(hipEventRecord(p->join[i], p->side[i]));
+          CSAW_HIP(hipStreamWaitEvent(st, p->join[i], 0));
+        }
+      } else {
+        CSAW_TRY(launch_expand_chunks(L, G, hv, u.d_req, u.d_gs2req, H, E, u.kinds, th));
+      }
       if (p->opts.log_chunks) {
         // keep this hop's chunks and log for the host
         p->hop_logs.emplace_back();
@@ -388,6 +410,13 @@ extern "C" int csaw_plan_create(csaw_graph* g, const csaw_request* reqs, int32_t
     }
     CSAW_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
     p->own_stream = true;
+    if (o.mode == CSAW_MODE_BATCHED && !o.serial_expand) {
+      CSAW_HIP(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
+      for (int i = 0; i < csaw_plan::kSide; ++i) {
+        CSAW_HIP(hipStreamCreateWithFlags(&p->side[i], hipStreamNonBlocking));
+        CSAW_HIP(hipEventCreateWithFlags(&p->join[i], hipEventDisableTiming));
+      }
+    }
     p->timer.enabled = o.timing != 0;
     return CSAW_OK;
   };

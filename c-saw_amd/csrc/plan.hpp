@@ -89,6 +89,12 @@ struct csaw_plan {
   bool own_stream = false;
   hipGraphExec_t gexec = nullptr;
   hipStream_t graph_stream = nullptr;
+  // batched mode: side streams the expansion kernels of a hop fan out to
+  // (joined back before the next hop), fork / join events
+  static constexpr int kSide = 3;
+  hipStream_t side[kSide] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[kSide] = {};
   csaw::EventTimer timer;
   double last_ms = 0.0;
   int64_t runs = 0;

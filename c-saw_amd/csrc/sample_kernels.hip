@@ -715,47 +715,61 @@ int launch_choose(const LaunchCfg& L, const ChooseArgs& A, const HopDev& H, int6
 
 int launch_expand_chunks(const LaunchCfg& L, const GraphDev& G, const csaw_graph_view& hv, const ReqDev* R,
                          const int32_t* gs2req, const HopDev& H, const ExpandArgs& A, const bool* kinds_present,
-                         TimingHook* th) {
+                         TimingHook* th, const hipStream_t* streams, int n_streams) {
+  int next = 0;
   auto go = [&](int which, auto kern, int per_cu) -> int {
+    const hipStream_t s = n_streams > 0 ? streams[next++ % n_streams] : L.stream;
     int grid = grid_of(L, per_cu);
-    if (th) th->begin(which, L.stream);
-    kern<<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, A);
+    if (th) th->begin(which, s);
+    kern<<<grid, 256, 0, s>>>(G, hv, R, gs2req, H, A);
     CSAW_LAUNCH_CHECK();
-    if (th) th->end(which, L.stream);
+    if (th) th->end(which, s);
     return CSAW_OK;
   };
-  CSAW_TRY(go(kKernCopy, k_expand_copy, L.expand_per_cu));
+  // integer caches of the view all uint32: the group-only builds
+  const bool g32 = A.int_cache && !G.int_cdf[0] && !G.int_cdf[1];
+  // WAVE kernels (most of the work) first, so that with several streams
+  // they start side by side
+  for (int kind = 0; kind < 3; ++kind) {
+    if (!kinds_present[kind]) continue;
+    if (kind == kUniform) {
+      CSAW_TRY(go(kKernWave, k_expand_wave<kUniform, true, true>, L.expand_per_cu));
+      CSAW_TRY(go(kKernWave, k_expand_wave<kUniform, false, true>, L.expand_per_cu));
+    } else if (kind == kInt) {
+      if (g32) {
+        CSAW_TRY(go(kKernWave, k_expand_wave<kInt, true, true>, L.expand_per_cu));
+        CSAW_TRY(go(kKernWave, k_expand_wave<kInt, false, true>, L.expand_per_cu));
+      } else {
+        CSAW_TRY(go(kKernWave, k_expand_wave<kInt, true, false>, L.expand_per_cu));
+        CSAW_TRY(go(kKernWave, k_expand_wave<kInt, false, false>, L.expand_per_cu));
+      }
+    } else {
+      CSAW_TRY(go(kKernWave, k_expand_wave<kF32, true, true>, L.expand_per_cu));
+      CSAW_TRY(go(kKernWave, k_expand_wave<kF32, false, true>, L.expand_per_cu));
+    }
+  }
   for (int kind = 0; kind < 3; ++kind) {
     if (!kinds_present[kind]) continue;
     if (kind == kUniform) {
       CSAW_TRY(go(kKernThread, k_expand_thread<kUniform>, L.thread_per_cu));
-      CSAW_TRY(go(kKernWave, k_expand_wave<kUniform, true, true>, L.expand_per_cu));
-      CSAW_TRY(go(kKernWave, k_expand_wave<kUniform, false, true>, L.expand_per_cu));
       CSAW_TRY(go(kKernBlock, k_expand_block<kUniform, true, true>, L.block_per_cu));
       CSAW_TRY(go(kKernBlock, k_expand_block<kUniform, false, true>, L.block_per_cu));
     } else if (kind == kInt) {
-      // group-only builds when every integer cache of the view is uint32
-      const bool g32 = A.int_cache && !G.int_cdf[0] && !G.int_cdf[1];
       CSAW_TRY(go(kKernThread, k_expand_thread<kInt>, L.thread_per_cu));
       if (g32) {
-        CSAW_TRY(go(kKernWave, k_expand_wave<kInt, true, true>, L.expand_per_cu));
-        CSAW_TRY(go(kKernWave, k_expand_wave<kInt, false, true>, L.expand_per_cu));
         CSAW_TRY(go(kKernBlock, k_expand_block<kInt, true, true>, L.block_per_cu));
         CSAW_TRY(go(kKernBlock, k_expand_block<kInt, false, true>, L.block_per_cu));
       } else {
-        CSAW_TRY(go(kKernWave, k_expand_wave<kInt, true, false>, L.expand_per_cu));
-        CSAW_TRY(go(kKernWave, k_expand_wave<kInt, false, false>, L.expand_per_cu));
         CSAW_TRY(go(kKernBlock, k_expand_block<kInt, true, false>, L.block_per_cu));
         CSAW_TRY(go(kKernBlock, k_expand_block<kInt, false, false>, L.block_per_cu));
       }
     } else {
       CSAW_TRY(go(kKernThread, k_expand_thread<kF32>, L.thread_per_cu));
-      CSAW_TRY(go(kKernWave, k_expand_wave<kF32, true, true>, L.expand_per_cu));
-      CSAW_TRY(go(kKernWave, k_expand_wave<kF32, false, true>, L.expand_per_cu));
       CSAW_TRY(go(kKernBlock, k_expand_block<kF32, true, true>, L.block_per_cu));
       CSAW_TRY(go(kKernBlock, k_expand_block<kF32, false, true>, L.block_per_cu));
     }
   }
+  CSAW_TRY(go(kKernCopy, k_expand_copy, L.expand_per_cu));
   return CSAW_OK;
 }
 

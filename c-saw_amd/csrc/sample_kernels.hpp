@@ -92,8 +92,10 @@ int launch_expand_plain(const LaunchCfg& L, int kind, const GraphDev& G, const c
 int launch_choose(const LaunchCfg& L, const ChooseArgs& A, const HopDev& H, int64_t max_chunks);
 int launch_expand_list(const LaunchCfg& L, int kind, const GraphDev& G, const csaw_graph_view& hv, const ReqDev* R,
                        const int32_t* gs2req, const HopDev& H, const int32_t* list, int64_t n);
+// The variant kernels of one hop; with n_streams > 0 kernel i goes to
+// streams[i % n_streams] (heaviest kinds first), else all to L.stream.
 int launch_expand_chunks(const LaunchCfg& L, const GraphDev& G, const csaw_graph_view& hv, const ReqDev* R,
                          const int32_t* gs2req, const HopDev& H, const ExpandArgs& A, const bool* kinds_present,
-                         TimingHook* th);
+                         TimingHook* th, const hipStream_t* streams = nullptr, int n_streams = 0);
 
 }  // namespace csaw

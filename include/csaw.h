@@ -231,6 +231,9 @@ typedef struct csaw_plan_opts {
   const struct csaw_tuning *tuning; /* launch shapes (or NULL: defaults)   */
   int32_t cdf_cache; /* integer-bias draws use the per-graph CDF cache (the */
                      /* reference's Cp, functions.cuh:329-341; off there)   */
+  int32_t serial_expand; /* batched: run a hop's variant kernels one after */
+                         /* another on one stream (default: side by side  */
+                         /* on four streams)                               */
 } csaw_plan_opts;
 
 typedef struct csaw_plan csaw_plan;
