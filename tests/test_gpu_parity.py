@@ -184,11 +184,16 @@ def test_batched_equals_plain_and_oracle(C, O, g14):
             same(bat.result(i), want, f"batched req {i}")
 
 
-def test_hipgraph_replays(C, O, g14):
+@pytest.mark.parametrize("cdf_cache,serial", [(False, False), (True, False), (True, True)])
+def test_hipgraph_replays(C, O, g14, cdf_cache, serial):
+    # the captured hops fork each hop's variant kernels onto side streams
+    # (unless serial) and join them; every replay must equal the oracle
     g, og, *_ = g14
-    specs = [{"seeds": g.seeds(300, request=i), "fanouts": [10, 5], "bias": i % 3, "request": i} for i in range(6)]
+    specs = [{"seeds": g.seeds(300, request=i), "fanouts": [10, 5] if i % 2 else [6, 4, 3], "bias": i % 3,
+              "request": i} for i in range(6)]
     want = [O.sample(og, s["seeds"], s["fanouts"], bias=s["bias"], request=i) for i, s in enumerate(specs)]
-    p = C.Plan(g, specs, mode=C.MODE_BATCHED, use_graph=True)
+    p = C.Plan(g, specs, mode=C.MODE_BATCHED, use_graph=True, cdf_cache=cdf_cache, serial_expand=serial,
+               sched=C.SCHED_RANDOM, explore_key=3)
     for rep in range(4):
         p.run()
         for i in range(len(specs)):
