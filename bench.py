@@ -302,6 +302,7 @@ def main():
     ap.add_argument("--workload", choices=["c4", "c1", "c2", "c3", "uniform", "c5"], default="c4",
                     help="c4 (default) is the headline; the others measure the remaining configs")
     ap.add_argument("--no-graph", action="store_true", help="--only batched: launch eagerly (profilers)")
+    ap.add_argument("--gather-mib", type=int, default=512, help="--only gather: array size (MiB, < 8192)")
     ap.add_argument("--serial-expand", action="store_true",
                     help="run each hop's variant kernels one after another on one stream")
     ap.add_argument("--only", choices=["all", "batched", "plain", "copy", "gather"], default="all",
@@ -323,9 +324,9 @@ def main():
         return
     if a.only == "gather":
         # random-read ceiling: 64M independent random 4-byte reads from a
-        # 512 MiB array (the size of the RMAT-22 neighbour array)
+        # 512 MiB array (the size of the RMAT-22 neighbour array) by default
         import torch
-        x = torch.ones(128 << 20, dtype=torch.int32, device="cuda")
+        x = torch.ones(min(a.gather_mib, 8191) << 18, dtype=torch.int32, device="cuda")
         idx = torch.randint(0, x.numel(), (64 << 20,), device="cuda", dtype=torch.int32)
         y = x[idx]
         torch.cuda.synchronize()
@@ -336,7 +337,8 @@ def main():
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / a.steps
-        print(json.dumps({"random_reads": idx.numel(), "ms": ms, "reads_per_s": idx.numel() / (ms * 1e-3),
+        print(json.dumps({"array_mib": x.numel() >> 18, "random_reads": idx.numel(), "ms": ms,
+                          "reads_per_s": idx.numel() / (ms * 1e-3),
                           "line_bytes_per_s_at_64B": idx.numel() * 64 / (ms * 1e-3)}))
         return
     import __graft_entry__ as ge

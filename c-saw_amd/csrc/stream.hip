@@ -153,12 +153,34 @@ __global__ __launch_bounds__(256) void k_frontier_cdf(GraphDev G, csaw_graph_vie
   }
 }
 
-__global__ void k_part_scatter(HopDev H, const uint8_t* __restrict__ pid, const uint32_t* __restrict__ part_off,
-                               uint32_t* __restrict__ cursor, int32_t* __restrict__ list) {
+// Entries grouped by partition into list (part_off[p] + slot): each
+// workgroup takes 1024 consecutive entries, ranks them per partition in LDS
+// and reserves each partition's run with one global atomic.
+__global__ __launch_bounds__(256) void k_part_scatter(HopDev H, const uint8_t* __restrict__ pid, int P,
+                                                      const uint32_t* __restrict__ part_off,
+                                                      uint32_t* __restrict__ cursor, int32_t* __restrict__ list) {
+  __shared__ uint32_t lcnt[256];
+  __shared__ uint32_t lbase[256];
   const int64_t n = *H.n_ptr;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-    const int p = pid[e];
-    list[part_off[p] + atomicAdd(&cursor[p], 1u)] = (int32_t)e;
+  for (int64_t b0 = blockIdx.x * 1024ll; b0 < n; b0 += (int64_t)gridDim.x * 1024) {
+    for (int i = threadIdx.x; i < P; i += blockDim.x) lcnt[i] = 0;
+    __syncthreads();
+    int pp[4];
+    uint32_t slot[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t e = b0 + k * 256 + threadIdx.x;
+      pp[k] = e < n ? (int)pid[e] : -1;
+      if (pp[k] >= 0) slot[k] = atomicAdd(&lcnt[pp[k]], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < P; i += blockDim.x)
+      if (lcnt[i]) lbase[i] = part_off[i] + atomicAdd(&cursor[i], lcnt[i]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (pp[k] >= 0) list[lbase[pp[k]] + slot[k]] = (int32_t)(b0 + k * 256 + threadIdx.x);
+    __syncthreads();
   }
 }
 
@@ -558,7 +580,7 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
       CSAW_HIP(hipStreamSynchronize(cs));
       for (int p = 0; p < P; ++p) poff[p + 1] = poff[p] + cnt[p];
       CSAW_HIP(hipMemcpyAsync(d_poff, poff.data(), sizeof(uint32_t) * P, hipMemcpyHostToDevice, cs));
-      k_part_scatter<<<grid_of(nmax[h], num_cu), 256, 0, cs>>>(H, d_pid, d_poff, d_cursor, d_list);
+      k_part_scatter<<<grid_of((nmax[h] + 3) / 4, num_cu), 256, 0, cs>>>(H, d_pid, P, d_poff, d_cursor, d_list);
       CSAW_LAUNCH_CHECK();
       // unique frontier hubs per partition (sorted by id)
       std::vector<std::vector<int32_t>> hubs(P);
@@ -623,7 +645,19 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
         return from;
       };
       size_t staged = next_copy(0);
-      if (staged < todo.size()) CSAW_TRY(stage(todo[staged], -1, &bufof[staged]));
+      if (staged < todo.size()) {
+        CSAW_TRY(stage(todo[staged], -1, &bufof[staged]));
+      } else if (h + 1 < r.depth && P > 1 && o.slice_policy != CSAW_SLICE_DIRECT) {
+        // a hop read entirely in place leaves both buffers idle: fill them
+        // with the next hop's first two partitions meanwhile (if that hop
+        // reads them in place instead, the copies were only wasted bus time)
+        const int q0 = ((h + 1) & 1) ? P - 1 : 0, q1 = ((h + 1) & 1) ? P - 2 : 1;
+        int b0 = -1, b1 = -1;
+        const int32_t keep = reused_slices;  // counted when the next hop finds them
+        CSAW_TRY(stage(q0, -1, &b0));
+        CSAW_TRY(stage(q1, b0, &b1));
+        reused_slices = keep;
+      }
       for (size_t t = 0; t < todo.size(); ++t) {
         const int p = todo[t];
         const int buf = direct[p] ? -1 : bufof[t];
