@@ -253,6 +253,10 @@ def bench_stream(a, C, O, CM):
                 ms[k].append(last[k]["stats"].device_ms)
                 ems[k].append(last[k]["stats"].expand_ms)
     log("per-run ms", {k: [round(x, 1) for x in v] for k, v in ms.items()})
+    for k in names:
+        st_k = last[k]["stats"]
+        log(k, "hop ms", [round(x, 1) for x in list(st_k.hop_ms)[:3]], "copy ms", round(st_k.copy_ms, 1),
+            "h2d GB", round(st_k.h2d_bytes / 1e9, 2), "direct", st_k.direct_partitions, "reused", st_k.reused_slices)
     res, heur, plain = last["mlp"], last["heuristic"], last["plain"]
     dms, ems_ = float(np.median(ms["mlp"])), float(np.median(ems["mlp"]))
     hms, hems = float(np.median(ms["heuristic"])), float(np.median(ems["heuristic"]))

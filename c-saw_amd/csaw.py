@@ -439,7 +439,8 @@ class StreamStats(ctypes.Structure):
     _fields_ = [("edges", i64), ("device_ms", ctypes.c_double), ("h2d_bytes", ctypes.c_double),
                 ("partitions", i32), ("peak_device_bytes", i64), ("slice_bytes", i64),
                 ("expand_ms", ctypes.c_double), ("chunks", i64), ("variant_chunks", i64 * NUM_VARIANTS),
-                ("direct_partitions", i32), ("reused_slices", i32)]
+                ("direct_partitions", i32), ("reused_slices", i32), ("copy_ms", ctypes.c_double),
+                ("hop_ms", ctypes.c_double * 8)]
 
 
 class StreamOpts(ctypes.Structure):

@@ -366,10 +366,12 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
   hipStream_t cs = nullptr, xs = nullptr;
   hipEvent_t ready[2] = {nullptr, nullptr}, freed[2] = {nullptr, nullptr}, t0 = nullptr, t1 = nullptr;
   std::vector<hipEvent_t> evs;  // per processed partition: bin begin/end, expand begin/end
+  std::vector<hipEvent_t> cev;  // per slice copy: begin/end on the copy stream
+  std::vector<hipEvent_t> hev;  // per hop: start on the compute stream
   uint32_t* h_qlen = nullptr;   // pinned: queue lengths per (hop, partition)
   std::vector<int32_t> h_src, h_dst, h_smp;
   std::vector<int64_t> h_ho;
-  double h2d_bytes = 0, h2d_ms = 0, expand_ms = 0;
+  double h2d_bytes = 0, h2d_ms = 0, expand_ms = 0, copy_ms = 0, hop_ms[CSAW_MAX_HOPS] = {};
   int64_t n_chunks_all = 0, var_chunks[CSAW_NUM_VARIANTS] = {};
   int32_t direct_parts = 0, reused_slices = 0;
   int64_t log_rows = 0;
@@ -538,6 +540,9 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
     };
     CSAW_HIP(hipEventRecord(t0, cs));
     for (int h = 0; h < r.depth; ++h) {
+      hev.push_back(nullptr);
+      CSAW_HIP(hipEventCreate(&hev.back()));
+      CSAW_HIP(hipEventRecord(hev.back(), cs));
       HopDev H;
       memset(&H, 0, sizeof(H));
       H.fv = h == 0 ? d_fv0 : d_dst[h - 1];
@@ -630,7 +635,13 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
                                  : (held[0] < 0 ? 0 : (held[1] < 0 ? 1 : (last_use[0] <= last_use[1] ? 0 : 1)));
         const int64_t a = hg->off[vb[p]], e = hg->off[vb[p + 1]];
         CSAW_HIP(hipStreamWaitEvent(xs, freed[b], 0));
+        cev.push_back(nullptr);
+        CSAW_HIP(hipEventCreate(&cev.back()));
+        CSAW_HIP(hipEventRecord(cev.back(), xs));
         if (e > a) CSAW_HIP(hipMemcpyAsync(slice[b], hg->adj + a, sizeof(int32_t) * (e - a), hipMemcpyHostToDevice, xs));
+        cev.push_back(nullptr);
+        CSAW_HIP(hipEventCreate(&cev.back()));
+        CSAW_HIP(hipEventRecord(cev.back(), xs));
         CSAW_HIP(hipEventRecord(ready[b], xs));
         h2d_bytes += (double)(e - a) * sizeof(int32_t);
         held[b] = p;
@@ -804,6 +815,17 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
       CSAW_HIP(hipEventElapsedTime(&m, evs[i], evs[i + 1]));
       expand_ms += m;
     }
+    CSAW_HIP(hipStreamSynchronize(xs));
+    for (size_t i = 0; i + 1 < cev.size(); i += 2) {
+      float m = 0.f;
+      CSAW_HIP(hipEventElapsedTime(&m, cev[i], cev[i + 1]));
+      copy_ms += m;
+    }
+    for (size_t i = 0; i < hev.size(); ++i) {
+      float m = 0.f;
+      CSAW_HIP(hipEventElapsedTime(&m, hev[i], i + 1 < hev.size() ? hev[i + 1] : t1));
+      hop_ms[i] = m;
+    }
     if (batched) {
       uint32_t flags = 0;
       CSAW_HIP(hipMemcpy(&flags, d_ctr + kCtrFlags, sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -853,6 +875,10 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
   if (t0) (void)hipEventDestroy(t0);
   if (t1) (void)hipEventDestroy(t1);
   for (hipEvent_t e : evs) (void)hipEventDestroy(e);
+  for (hipEvent_t e : cev)
+    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : hev)
+    if (e) (void)hipEventDestroy(e);
   if (cs) (void)hipStreamSynchronize(cs);
   if (xs) (void)hipStreamSynchronize(xs);
   if (cs) (void)hipStreamDestroy(cs);
@@ -869,6 +895,8 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
     stats->peak_device_bytes = B.peak;
     stats->slice_bytes = slice_max * (int64_t)sizeof(int32_t);
     stats->expand_ms = expand_ms;
+    stats->copy_ms = copy_ms;
+    for (int h = 0; h < CSAW_MAX_HOPS; ++h) stats->hop_ms[h] = hop_ms[h];
     stats->direct_partitions = direct_parts;
     stats->reused_slices = reused_slices;
     stats->chunks = n_chunks_all;

@@ -302,6 +302,8 @@ typedef struct csaw_stream_stats {
   int64_t variant_chunks[CSAW_NUM_VARIANTS]; /* chunks run per variant      */
   int32_t direct_partitions; /* (hop, partition) pairs read in place        */
   int32_t reused_slices;     /* slices a hop found left on the device       */
+  double copy_ms;            /* time the copy stream spent in slice copies  */
+  double hop_ms[CSAW_MAX_HOPS]; /* device time of each hop                  */
 } csaw_stream_stats;
 
 /* Where a partition's adjacency comes from in a hop. */
