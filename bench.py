@@ -234,6 +234,10 @@ def bench_stream(a, C, O, CM):
     t0 = time.time()
     hg = C.HostGraph.rmat(26, 16)
     log(f"host graph V={hg.V} E={hg.E} maxdeg={hg.max_degree} ({time.time() - t0:.1f}s)")
+    t0 = time.time()
+    packed = hg.pack() if a.packed_slices else 0
+    if packed:
+        log(f"packed adjacency {packed / 1e9:.2f} GB of {4 * hg.E / 1e9:.2f} GB ({time.time() - t0:.1f}s)")
     mlp, meta = get_stream_model(C, CM, hg, a.retrain)
     seeds = C.seeds_v(hg.V, 65536)
     kw = dict(bias=C.BIAS_DEGREE, budget_bytes=4 << 30, partitions=16)
@@ -284,7 +288,8 @@ def bench_stream(a, C, O, CM):
          {"workload": "RMAT-26 ef16 host-resident, 4 GB budget, 16 partitions, 65,536 seeds, {15,10,5}, degree",
           "schedule": "learned cost model per chunk", "edges": int(st.edges), "h2d_bytes": st.h2d_bytes,
           "peak_device_bytes": int(st.peak_device_bytes), "slice_bytes": int(st.slice_bytes),
-          "direct_partitions": int(st.direct_partitions), "reused_slices": int(st.reused_slices)}, "u64", None, cpu,
+          "direct_partitions": int(st.direct_partitions), "reused_slices": int(st.reused_slices),
+          "packed_adjacency_bytes": int(packed), "h2d_raw_bytes": st.h2d_raw_bytes}, "u64", None, cpu,
          h2d_gbs=st.h2d_bytes / (dms * 1e-3) / 1e9,
          schedules={"mlp": {"ms": dms, "expand_ms": ems, "variant_chunks": list(st.variant_chunks),
                             "chunks": int(st.chunks)},
@@ -307,6 +312,8 @@ def main():
                     help="c4 (default) is the headline; the others measure the remaining configs")
     ap.add_argument("--no-graph", action="store_true", help="--only batched: launch eagerly (profilers)")
     ap.add_argument("--gather-mib", type=int, default=512, help="--only gather: array size (MiB, < 8192)")
+    ap.add_argument("--packed-slices", action="store_true",
+                    help="c5: pack the host adjacency and copy packed slices (fewer bytes, unpacked on the device)")
     ap.add_argument("--serial-expand", action="store_true",
                     help="run each hop's variant kernels one after another on one stream")
     ap.add_argument("--only", choices=["all", "batched", "plain", "copy", "gather"], default="all",

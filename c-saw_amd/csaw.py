@@ -440,12 +440,12 @@ class StreamStats(ctypes.Structure):
                 ("partitions", i32), ("peak_device_bytes", i64), ("slice_bytes", i64),
                 ("expand_ms", ctypes.c_double), ("chunks", i64), ("variant_chunks", i64 * NUM_VARIANTS),
                 ("direct_partitions", i32), ("reused_slices", i32), ("copy_ms", ctypes.c_double),
-                ("hop_ms", ctypes.c_double * 8)]
+                ("h2d_raw_bytes", ctypes.c_double), ("hop_ms", ctypes.c_double * 8)]
 
 
 class StreamOpts(ctypes.Structure):
     _fields_ = [("mode", i32), ("sched", i32), ("variant", i32), ("explore_key", u64), ("mlp", P), ("tuning", P),
-                ("slice_policy", i32), ("log_chunks", i32), ("log_cap", i64), ("log_features", P),
+                ("slice_policy", i32), ("raw_slices", i32), ("log_chunks", i32), ("log_cap", i64), ("log_features", P),
                 ("log_variant", P), ("log_log2_ticks", P), ("log_keys", P), ("log_n", P)]
 
 
@@ -454,6 +454,7 @@ _sig("csaw_host_graph_create_rmat", [ctypes.c_int, ctypes.c_int, ctypes.c_double
 _sig("csaw_host_graph_from_csr", [P, P, i64, i64, P])
 _sig("csaw_host_graph_info", [P, P, P, P, P, P])
 _sig("csaw_host_graph_destroy", [P])
+_sig("csaw_host_graph_pack", [P, i32, P])
 _sig("csaw_stream_sample", [P, P, i64, i32, P, P])
 _sig("csaw_stream_sample_opts", [P, P, i64, i32, P, P, P])
 _sig("csaw_edges_from_host", [i32, i32, i64, P, P, P, P, P])
@@ -487,6 +488,12 @@ class HostGraph:
                "host_graph_from_csr")
         return cls(h)
 
+    def pack(self, threads=0):
+        """Packed copy of the neighbour ids for the streamed slice copies; returns its bytes."""
+        b = i64()
+        _check(_L.csaw_host_graph_pack(self.h, threads, ctypes.byref(b)), "host_graph_pack")
+        return b.value
+
     def csr_views(self):
         """Zero-copy numpy views of the pinned CSR (valid while the graph lives)."""
         off = np.ctypeslib.as_array(ctypes.cast(self._po, ctypes.POINTER(ctypes.c_int64)), (self.V + 1,))
@@ -507,7 +514,7 @@ class HostGraph:
 
 def stream_sample(hg, seeds, fanouts, bias=BIAS_DEGREE, dedup=1, key=42, request=0, budget_bytes=4 << 30,
                   partitions=16, mode=MODE_BATCHED, sched=SCHED_HEURISTIC, variant=0, mlp=None, explore_key=0,
-                  tuning=None, slice_policy=0, log_cap=0):
+                  tuning=None, slice_policy=0, log_cap=0, raw_slices=False):
     """Streams one request; mode / sched / variant / mlp choose how each
     partition's entries are expanded (the edge list is the same for all).
     log_cap > 0 also returns the per-chunk cost-model rows as "log":
@@ -522,6 +529,7 @@ def stream_sample(hg, seeds, fanouts, bias=BIAS_DEGREE, dedup=1, key=42, request
         logs = (np.zeros((log_cap, 16), np.float32), np.zeros(log_cap, np.int32), np.zeros(log_cap, np.float32),
                 np.zeros(log_cap, np.uint64))
     o = StreamOpts(mode=mode, sched=sched, variant=variant, explore_key=explore_key, slice_policy=slice_policy,
+                   raw_slices=1 if raw_slices else 0,
                    log_chunks=1 if logs else 0, log_cap=log_cap,
                    log_features=_p(logs[0]) if logs else None, log_variant=_p(logs[1]) if logs else None,
                    log_log2_ticks=_p(logs[2]) if logs else None, log_keys=_p(logs[3]) if logs else None,

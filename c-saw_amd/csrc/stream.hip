@@ -26,9 +26,11 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "graph.hpp"
+#include "pack.hpp"
 #include "plan.hpp"
 #include "sample_kernels.hpp"
 
@@ -36,6 +38,12 @@ struct csaw_host_graph {
   int64_t V = 0, E = 0, max_deg = 0;
   int64_t* off = nullptr;  // pinned host, V+1
   int32_t* adj = nullptr;  // pinned host, E
+  // packed copy of adj (csaw_host_graph_pack): blocks of 64 consecutive
+  // entries, block b at byte boff[b] of pdata
+  uint8_t* pdata = nullptr;     // pinned
+  uint32_t* boff32 = nullptr;   // pinned, low 32 bits of boff (nblk + 1)
+  std::vector<uint64_t> boff;   // nblk + 1
+  int64_t nblk = 0;
 };
 
 namespace csaw {
@@ -184,6 +192,35 @@ __global__ __launch_bounds__(256) void k_part_scatter(HopDev H, const uint8_t* _
   }
 }
 
+constexpr int64_t kPieceBlocks = 1 << 17;  // packed blocks per staged piece (8M entries)
+
+// One wave per block of a staged piece: blocks pb0 + k (k < nb) at bytes
+// (off[k] - off[0]) of data; entries outside [a, e) are not written; the
+// slice holds entry i at slice[i - a].
+__global__ __launch_bounds__(256) void k_unpack(const uint8_t* __restrict__ data, const uint32_t* __restrict__ off,
+                                                int64_t pb0, int64_t nb, int64_t E, int64_t a, int64_t e,
+                                                int32_t* __restrict__ slice) {
+  const int lane = lane_id();
+  for (int64_t k = blockIdx.x * 4 + threadIdx.x / kWave; k < nb; k += (int64_t)gridDim.x * 4) {
+    const uint8_t* blk = data + (uint32_t)(off[k] - off[0]);
+    const int64_t first = (pb0 + k) * kPackBlock;
+    const int n = (int)min((int64_t)kPackBlock, E - first);
+    const int w = blk[0];
+    uint32_t v = 0;
+    if (lane == 0) {
+      v = (uint32_t)blk[1] | ((uint32_t)blk[2] << 8) | ((uint32_t)blk[3] << 16) | ((uint32_t)blk[4] << 24);
+    } else if (lane < n) {
+      const uint8_t* q = blk + 5 + (lane - 1) * w;
+      uint32_t z = 0;
+      for (int i = 0; i < w; ++i) z |= (uint32_t)q[i] << (8 * i);
+      v = (z >> 1) ^ (0u - (z & 1u));  // zigzag decode
+    }
+    const uint32_t x = wave_incl_scan_u32(v);  // ids: first + running differences (mod 2^32)
+    const int64_t i = first + lane;
+    if (lane < n && i >= a && i < e) slice[i - a] = (int32_t)x;
+  }
+}
+
 struct Budget {
   int64_t limit, used = 0, peak = 0;
   std::vector<void*> ptrs;
@@ -295,7 +332,51 @@ extern "C" int csaw_host_graph_destroy(csaw_host_graph* g) {
   if (!g) return CSAW_OK;
   (void)hipHostFree(g->off);
   (void)hipHostFree(g->adj);
+  if (g->pdata) (void)hipHostFree(g->pdata);
+  if (g->boff32) (void)hipHostFree(g->boff32);
   delete g;
+  return CSAW_OK;
+}
+
+extern "C" int csaw_host_graph_pack(csaw_host_graph* g, int32_t threads, int64_t* packed_bytes) {
+  if (!g || threads < 0) return CSAW_ERR_INVALID_ARG;
+  if (!g->pdata) {
+    const int64_t nblk = (g->E + kPackBlock - 1) / kPackBlock;
+    const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(threads > 0 ? threads : 16, std::max<int64_t>(nblk, 1)));
+    std::vector<uint64_t> boff(nblk + 1, 0);
+    auto for_blocks = [&](auto fn) {
+      std::vector<std::thread> ts;
+      for (int t = 0; t < nt; ++t)
+        ts.emplace_back([&, t]() {
+          for (int64_t b = nblk * t / nt; b < nblk * (t + 1) / nt; ++b) fn(b);
+        });
+      for (auto& th : ts) th.join();
+    };
+    for_blocks([&](int64_t b) {
+      const int64_t i0 = b * kPackBlock;
+      boff[b + 1] = (uint64_t)pack_block(g->adj + i0, (int)std::min<int64_t>(kPackBlock, g->E - i0), nullptr);
+    });
+    for (int64_t b = 0; b < nblk; ++b) boff[b + 1] += boff[b];
+    uint8_t* pdata = nullptr;
+    uint32_t* b32 = nullptr;
+    if (hipHostMalloc((void**)&pdata, std::max<uint64_t>(boff[nblk], 1), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&b32, sizeof(uint32_t) * (nblk + 1), hipHostMallocDefault) != hipSuccess) {
+      if (pdata) (void)hipHostFree(pdata);
+      set_last_error("cannot pin memory for the packed adjacency");
+      return CSAW_ERR_OOM;
+    }
+    for_blocks([&](int64_t b) {
+      const int64_t i0 = b * kPackBlock;
+      pack_block(g->adj + i0, (int)std::min<int64_t>(kPackBlock, g->E - i0), pdata + boff[b]);
+      b32[b] = (uint32_t)boff[b];
+    });
+    b32[nblk] = (uint32_t)boff[nblk];
+    g->pdata = pdata;
+    g->boff32 = b32;
+    g->boff = std::move(boff);
+    g->nblk = nblk;
+  }
+  if (packed_bytes) *packed_bytes = (int64_t)g->boff[g->nblk] + 4 * (g->nblk + 1);
   return CSAW_OK;
 }
 
@@ -371,13 +452,25 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
   uint32_t* h_qlen = nullptr;   // pinned: queue lengths per (hop, partition)
   std::vector<int32_t> h_src, h_dst, h_smp;
   std::vector<int64_t> h_ho;
-  double h2d_bytes = 0, h2d_ms = 0, expand_ms = 0, copy_ms = 0, hop_ms[CSAW_MAX_HOPS] = {};
+  double h2d_bytes = 0, h2d_raw_bytes = 0, h2d_ms = 0, expand_ms = 0, copy_ms = 0, hop_ms[CSAW_MAX_HOPS] = {};
+  const bool packed = hg->pdata != nullptr && !o.raw_slices;
+  hipStream_t us = nullptr;  // unpack stream (packed slices)
+  hipEvent_t staged[2] = {nullptr, nullptr}, consumed[2] = {nullptr, nullptr};
+  int64_t piece_tick = 0;
   int64_t n_chunks_all = 0, var_chunks[CSAW_NUM_VARIANTS] = {};
   int32_t direct_parts = 0, reused_slices = 0;
   int64_t log_rows = 0;
   auto body = [&]() -> int {
     CSAW_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     CSAW_HIP(hipStreamCreateWithFlags(&xs, hipStreamNonBlocking));
+    if (packed) {
+      CSAW_HIP(hipStreamCreateWithFlags(&us, hipStreamNonBlocking));
+      for (int i = 0; i < 2; ++i) {
+        CSAW_HIP(hipEventCreateWithFlags(&staged[i], hipEventDisableTiming));
+        CSAW_HIP(hipEventCreateWithFlags(&consumed[i], hipEventDisableTiming));
+        CSAW_HIP(hipEventRecord(consumed[i], us));  // both staging buffers start free
+      }
+    }
     for (int i = 0; i < 2; ++i) {
       CSAW_HIP(hipEventCreateWithFlags(&ready[i], hipEventDisableTiming));
       CSAW_HIP(hipEventCreateWithFlags(&freed[i], hipEventDisableTiming));
@@ -395,6 +488,14 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
     int32_t* slice[2];
     CSAW_TRY(B.alloc(&slice[0], slice_max));
     CSAW_TRY(B.alloc(&slice[1], slice_max));
+    uint8_t* sdata[2] = {nullptr, nullptr};
+    uint32_t* soff[2] = {nullptr, nullptr};
+    if (packed) {
+      for (int i = 0; i < 2; ++i) {
+        CSAW_TRY(B.alloc(&sdata[i], kPieceBlocks * kPackMaxBlockBytes));
+        CSAW_TRY(B.alloc(&soff[i], kPieceBlocks + 1));
+      }
+    }
     int held[2] = {-1, -1};  // partition whose slice each buffer holds
     int64_t last_use[2] = {0, 0}, use_tick = 0;
     // the pinned host adjacency as the device addresses it, for partitions
@@ -625,7 +726,9 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
       }
       // *out = a buffer, other than avoid, holding partition p (the copy is
       // issued when neither buffer holds it)
-      auto stage = [&](int p, int avoid, int* out) -> int {
+      // raw: copy unpacked even when the graph is packed (unpacking needs
+      // CUs, which a hop read in place is using)
+      auto stage = [&](int p, int avoid, int* out, bool raw = false) -> int {
         if ((held[0] == p && avoid != 0) || (held[1] == p && avoid != 1)) {
           *out = held[0] == p && avoid != 0 ? 0 : 1;
           reused_slices += 1;
@@ -634,16 +737,47 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
         const int b = avoid >= 0 ? 1 - avoid
                                  : (held[0] < 0 ? 0 : (held[1] < 0 ? 1 : (last_use[0] <= last_use[1] ? 0 : 1)));
         const int64_t a = hg->off[vb[p]], e = hg->off[vb[p + 1]];
-        CSAW_HIP(hipStreamWaitEvent(xs, freed[b], 0));
-        cev.push_back(nullptr);
-        CSAW_HIP(hipEventCreate(&cev.back()));
-        CSAW_HIP(hipEventRecord(cev.back(), xs));
-        if (e > a) CSAW_HIP(hipMemcpyAsync(slice[b], hg->adj + a, sizeof(int32_t) * (e - a), hipMemcpyHostToDevice, xs));
-        cev.push_back(nullptr);
-        CSAW_HIP(hipEventCreate(&cev.back()));
-        CSAW_HIP(hipEventRecord(cev.back(), xs));
-        CSAW_HIP(hipEventRecord(ready[b], xs));
-        h2d_bytes += (double)(e - a) * sizeof(int32_t);
+        auto mark = [&]() -> int {  // copy-stream timing
+          cev.push_back(nullptr);
+          CSAW_HIP(hipEventCreate(&cev.back()));
+          CSAW_HIP(hipEventRecord(cev.back(), xs));
+          return CSAW_OK;
+        };
+        if (!packed || raw) {
+          CSAW_HIP(hipStreamWaitEvent(xs, freed[b], 0));
+          CSAW_TRY(mark());
+          if (e > a)
+            CSAW_HIP(hipMemcpyAsync(slice[b], hg->adj + a, sizeof(int32_t) * (e - a), hipMemcpyHostToDevice, xs));
+          CSAW_TRY(mark());
+          CSAW_HIP(hipEventRecord(ready[b], xs));
+          h2d_bytes += (double)(e - a) * sizeof(int32_t);
+        } else {
+          // packed blocks covering [a, e), in pieces through two staging
+          // buffers: piece j+1 crosses the bus while piece j is unpacked
+          // into the slice buffer (on the unpack stream, after the buffer's
+          // previous partition has expanded)
+          CSAW_HIP(hipStreamWaitEvent(us, freed[b], 0));
+          const int64_t b0 = a / kPackBlock, b1 = (e + kPackBlock - 1) / kPackBlock;
+          for (int64_t pb = b0; pb < b1; pb += kPieceBlocks) {
+            const int64_t nb = std::min<int64_t>(kPieceBlocks, b1 - pb);
+            const int sb = (int)(piece_tick++ & 1);
+            const uint64_t d0 = hg->boff[pb], d1 = hg->boff[pb + nb];
+            CSAW_HIP(hipStreamWaitEvent(xs, consumed[sb], 0));
+            CSAW_TRY(mark());
+            CSAW_HIP(hipMemcpyAsync(soff[sb], hg->boff32 + pb, sizeof(uint32_t) * (nb + 1), hipMemcpyHostToDevice, xs));
+            CSAW_HIP(hipMemcpyAsync(sdata[sb], hg->pdata + d0, d1 - d0, hipMemcpyHostToDevice, xs));
+            CSAW_TRY(mark());
+            CSAW_HIP(hipEventRecord(staged[sb], xs));
+            CSAW_HIP(hipStreamWaitEvent(us, staged[sb], 0));
+            const int g = (int)std::max<int64_t>(1, std::min<int64_t>((nb + 3) / 4, (int64_t)num_cu * 8));
+            k_unpack<<<g, 256, 0, us>>>(sdata[sb], soff[sb], pb, nb, E, a, e, slice[b]);
+            CSAW_LAUNCH_CHECK();
+            CSAW_HIP(hipEventRecord(consumed[sb], us));
+            h2d_bytes += (double)(d1 - d0) + 4.0 * (double)(nb + 1);
+          }
+          CSAW_HIP(hipEventRecord(ready[b], us));
+        }
+        h2d_raw_bytes += (double)(e - a) * sizeof(int32_t);
         held[b] = p;
         *out = b;
         return CSAW_OK;
@@ -665,8 +799,8 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
         const int q0 = ((h + 1) & 1) ? P - 1 : 0, q1 = ((h + 1) & 1) ? P - 2 : 1;
         int b0 = -1, b1 = -1;
         const int32_t keep = reused_slices;  // counted when the next hop finds them
-        CSAW_TRY(stage(q0, -1, &b0));
-        CSAW_TRY(stage(q1, b0, &b1));
+        CSAW_TRY(stage(q0, -1, &b0, true));
+        CSAW_TRY(stage(q1, b0, &b1, true));
         reused_slices = keep;
       }
       for (size_t t = 0; t < todo.size(); ++t) {
@@ -816,6 +950,7 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
       expand_ms += m;
     }
     CSAW_HIP(hipStreamSynchronize(xs));
+    if (us) CSAW_HIP(hipStreamSynchronize(us));
     for (size_t i = 0; i + 1 < cev.size(); i += 2) {
       float m = 0.f;
       CSAW_HIP(hipEventElapsedTime(&m, cev[i], cev[i + 1]));
@@ -881,8 +1016,14 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
     if (e) (void)hipEventDestroy(e);
   if (cs) (void)hipStreamSynchronize(cs);
   if (xs) (void)hipStreamSynchronize(xs);
+  if (us) (void)hipStreamSynchronize(us);
+  for (int i = 0; i < 2; ++i) {
+    if (staged[i]) (void)hipEventDestroy(staged[i]);
+    if (consumed[i]) (void)hipEventDestroy(consumed[i]);
+  }
   if (cs) (void)hipStreamDestroy(cs);
   if (xs) (void)hipStreamDestroy(xs);
+  if (us) (void)hipStreamDestroy(us);
   if (h_qlen) (void)hipHostFree(h_qlen);
   if (st != CSAW_OK) return st;
   if (o.log_n) *o.log_n = log_rows;
@@ -896,6 +1037,7 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
     stats->slice_bytes = slice_max * (int64_t)sizeof(int32_t);
     stats->expand_ms = expand_ms;
     stats->copy_ms = copy_ms;
+    stats->h2d_raw_bytes = h2d_raw_bytes;
     for (int h = 0; h < CSAW_MAX_HOPS; ++h) stats->hop_ms[h] = hop_ms[h];
     stats->direct_partitions = direct_parts;
     stats->reused_slices = reused_slices;

@@ -287,6 +287,11 @@ int csaw_host_graph_from_csr(const int64_t *offsets, const int32_t *indices, int
 int csaw_host_graph_info(const csaw_host_graph *g, int64_t *V, int64_t *E, int64_t *max_degree,
                          const int64_t **offsets, const int32_t **indices);
 int csaw_host_graph_destroy(csaw_host_graph *g);
+/* Builds a packed copy of the neighbour ids in pinned memory (blocks of 64
+ * entries: first id + zigzag differences in 1-4 bytes each), encoded on
+ * `threads` host threads (0: 16); streamed runs then copy packed slices and
+ * unpack them on the device.  packed_bytes: bytes of the packed copy. */
+int csaw_host_graph_pack(csaw_host_graph *g, int32_t threads, int64_t *packed_bytes);
 
 typedef struct csaw_stream_stats {
   int64_t edges;
@@ -303,6 +308,7 @@ typedef struct csaw_stream_stats {
   int32_t direct_partitions; /* (hop, partition) pairs read in place        */
   int32_t reused_slices;     /* slices a hop found left on the device       */
   double copy_ms;            /* time the copy stream spent in slice copies  */
+  double h2d_raw_bytes;      /* bytes the copied slices hold unpacked       */
   double hop_ms[CSAW_MAX_HOPS]; /* device time of each hop                  */
 } csaw_stream_stats;
 
@@ -327,6 +333,7 @@ typedef struct csaw_stream_opts {
   const struct csaw_mlp *mlp;       /* cost model for CSAW_SCHED_MLP or NULL */
   const struct csaw_tuning *tuning; /* launch shapes or NULL (defaults)      */
   int32_t slice_policy; /* csaw_slice_policy                                 */
+  int32_t raw_slices;   /* copy unpacked slices even if the graph is packed  */
   /* Per-chunk cost-model data (batched mode; each partition's chunks are
    * read back after it expands, so logging runs slower).  Rows as in
    * csaw_plan_chunk_log; keys are hop << 40 | partition << 32 | chunk key. */

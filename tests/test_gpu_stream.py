@@ -207,3 +207,41 @@ def test_stream_learned_scheduler_beats_heuristic(C):
     print("streaming held-out regret", r)
     assert r["chunks"] > 100
     assert r["mlp"] < r["heuristic"], r
+
+
+@pytest.mark.parametrize("bias", [0, 1, 3])
+def test_stream_packed_slices(C, graphs, bias):
+    # slices copied packed and unpacked on the device give the same edges as
+    # unpacked copies and as the in-memory sampler
+    g, _, _ = graphs
+    off, adj = g.csr()
+    hg = C.HostGraph.from_csr(off, adj)
+    packed = hg.pack(threads=4)
+    assert 0 < packed < 4 * len(adj)
+    seeds = g.seeds(1500, request=30 + bias)
+    want = C.sample(g, seeds, [15, 10, 5], bias=bias, request=6)
+    for raw in (False, True):
+        for name, kw in (("plain", dict(mode=C.MODE_PLAIN)), ("heuristic", dict()),
+                         ("forced1", dict(sched=C.SCHED_FORCED, variant=1))):
+            got = C.stream_sample(hg, seeds, [15, 10, 5], bias=bias, request=6, partitions=5, budget_bytes=1 << 30,
+                                  slice_policy=C.SLICE_COPY, raw_slices=raw, **kw)
+            same(got, want, f"bias {bias} raw {raw} {name}")
+            st = got["stats"]
+            if raw:
+                assert st.h2d_bytes == st.h2d_raw_bytes
+            else:
+                assert 0 < st.h2d_bytes < st.h2d_raw_bytes
+
+
+def test_stream_packed_multi_piece(C):
+    # partitions of ~16M entries: each slice crosses in two staged pieces
+    g = C.Graph.rmat(20, 16)
+    off, adj = g.csr()
+    hg = C.HostGraph.from_csr(off, adj)
+    hg.pack()
+    seeds = g.seeds(3000, request=9)
+    want = C.sample(g, seeds, [10, 4], bias=C.BIAS_DEGREE, request=9)
+    got = C.stream_sample(hg, seeds, [10, 4], bias=C.BIAS_DEGREE, request=9, partitions=2, budget_bytes=2 << 30,
+                          slice_policy=C.SLICE_COPY)
+    same(got, want, "two pieces per slice")
+    assert got["stats"].h2d_bytes < got["stats"].h2d_raw_bytes
