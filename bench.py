@@ -321,7 +321,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported "
-                                   "(the path shards by seeds with no exchange; see DESIGN.md)"}))
+                                   "(the path would shard by seeds with no exchange between GPUs)"}))
         sys.exit(2)
     if a.only == "copy":
         import torch
