@@ -55,8 +55,8 @@ __global__ void k_stream_degree(const int64_t* __restrict__ off, int64_t V, int3
 
 // Host bytes one frontier entry would read if its partition's adjacency
 // were read in place over the bus instead of copied: a 64-byte request per
-// picked neighbour, plus the neighbour list itself when an integer CDF is
-// built from it (an upper bound: a vertex's CDF is built once per partition).
+// picked neighbour, plus, for the first entry of each distinct vertex, the
+// neighbour list an integer CDF is built from.
 constexpr int64_t kBusRead = 64;
 constexpr int64_t kDirectFactor = 2;  // direct reads when 2x that estimate < the slice
 
@@ -92,10 +92,11 @@ __global__ __launch_bounds__(256) void k_part_bin(GraphDev G, const ReqDev* __re
     const bool picks = d > 0 && F > 0 && !(r.dedup && d <= F);
     unsigned long long eb = (unsigned long long)(kBusRead * entry_k(d, F, r.dedup));
     if (int_bias && picks) {
-      eb += 4ull * (unsigned long long)d;
+      // the CDF build reads the list once per distinct vertex: counted by
+      // the entry that sets the vertex's mark
       uint32_t* bits = d > kHubSeg ? hubbits : fbits;
       const uint32_t bit = 1u << (v & 31);
-      if (!(bits[v >> 5] & bit)) atomicOr(&bits[v >> 5], bit);
+      if (!(bits[v >> 5] & bit) && !(atomicOr(&bits[v >> 5], bit) & bit)) eb += 4ull * (unsigned long long)d;
     }
     atomicAdd(&lbytes[lo], eb);
   }
