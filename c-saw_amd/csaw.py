@@ -445,7 +445,8 @@ class StreamStats(ctypes.Structure):
 
 class StreamOpts(ctypes.Structure):
     _fields_ = [("mode", i32), ("sched", i32), ("variant", i32), ("explore_key", u64), ("mlp", P), ("tuning", P),
-                ("slice_policy", i32), ("raw_slices", i32), ("log_chunks", i32), ("log_cap", i64), ("log_features", P),
+                ("slice_policy", i32), ("raw_slices", i32), ("direct_factor", ctypes.c_float),
+                ("bin_min_entries", i32), ("log_chunks", i32), ("log_cap", i64), ("log_features", P),
                 ("log_variant", P), ("log_log2_ticks", P), ("log_keys", P), ("log_n", P)]
 
 
@@ -514,7 +515,7 @@ class HostGraph:
 
 def stream_sample(hg, seeds, fanouts, bias=BIAS_DEGREE, dedup=1, key=42, request=0, budget_bytes=4 << 30,
                   partitions=16, mode=MODE_BATCHED, sched=SCHED_HEURISTIC, variant=0, mlp=None, explore_key=0,
-                  tuning=None, slice_policy=0, log_cap=0, raw_slices=False):
+                  tuning=None, slice_policy=0, log_cap=0, raw_slices=False, direct_factor=0.0, bin_min_entries=0):
     """Streams one request; mode / sched / variant / mlp choose how each
     partition's entries are expanded (the edge list is the same for all).
     log_cap > 0 also returns the per-chunk cost-model rows as "log":
@@ -529,7 +530,7 @@ def stream_sample(hg, seeds, fanouts, bias=BIAS_DEGREE, dedup=1, key=42, request
         logs = (np.zeros((log_cap, 16), np.float32), np.zeros(log_cap, np.int32), np.zeros(log_cap, np.float32),
                 np.zeros(log_cap, np.uint64))
     o = StreamOpts(mode=mode, sched=sched, variant=variant, explore_key=explore_key, slice_policy=slice_policy,
-                   raw_slices=1 if raw_slices else 0,
+                   raw_slices=1 if raw_slices else 0, direct_factor=direct_factor, bin_min_entries=bin_min_entries,
                    log_chunks=1 if logs else 0, log_cap=log_cap,
                    log_features=_p(logs[0]) if logs else None, log_variant=_p(logs[1]) if logs else None,
                    log_log2_ticks=_p(logs[2]) if logs else None, log_keys=_p(logs[3]) if logs else None,

@@ -44,14 +44,17 @@ __host__ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
 
 __host__ __device__ __forceinline__ float flog2p1(float x) { return log2f(x + 1.0f); }
 
-// Features of one chunk.  kind: 0 uniform, 1 integer, 2 fp32.
+// Features of one chunk.  kind: 0 uniform, 1 integer, 2 fp32.  in_place:
+// the chunk's neighbour lists are read from host memory over the bus (a
+// streamed partition read in place); it adds 1 to the constant input f[0],
+// so that models trained without such chunks see f[0] = 1 as before.
 __host__ __device__ __forceinline__ void chunk_features(uint32_t len, float sum_d, uint32_t max_d, uint32_t sum_f,
                                                        uint32_t max_f, int kind, int hop, int bucket,
-                                                       uint32_t n_chunks, float* f) {
+                                                       uint32_t n_chunks, float* f, int in_place = 0) {
   const float n = (float)(len > 0 ? len : 1);
   const float mean_d = sum_d / n;
   const float mean_f = (float)sum_f / n;
-  f[0] = 1.0f;
+  f[0] = in_place ? 2.0f : 1.0f;
   f[1] = flog2p1(n) / 10.0f;
   f[2] = flog2p1(mean_d) / 20.0f;
   f[3] = flog2p1((float)max_d) / 20.0f;

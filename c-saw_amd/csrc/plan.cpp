@@ -265,6 +265,7 @@ static int run_unit(csaw_plan* p, Unit& u, hipStream_t st, TimingHook* th) {
       A.queue_cap = p->queue_cap;
       A.chosen = p->chosen;
       A.int_cache = p->opts.cdf_cache != 0;
+      A.in_place = 0;
       A.mlp.w = p->d_mlp;
       A.mlp_w = p->d_mlp;
       if (th) th->begin(kKernChoose, st);

@@ -357,7 +357,7 @@ __device__ int heuristic_variant(int kind, const Chunk& c, bool cache) {
 // (wl: the BF16 weights widened to FP32 in LDS).
 __device__ int mlp_choose(const ChooseArgs& A, const float* wl, const Chunk& c, int kind, const HopDev& H) {
   float f[kMlpIn], o[kMlpOut];
-  chunk_features(c.len, c.sum_d, c.max_d, c.sum_f, c.max_f, kind, c.hop, c.bucket, *H.n_chunks, f);
+  chunk_features(c.len, c.sum_d, c.max_d, c.sum_f, c.max_f, kind, c.hop, c.bucket, *H.n_chunks, f, A.in_place);
   mlp_forward1_f32(wl, f, o);
   int best = -1;
   float bv = 3.0e38f;

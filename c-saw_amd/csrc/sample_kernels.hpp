@@ -65,6 +65,7 @@ struct ChooseArgs {
   int64_t queue_cap;
   uint8_t* chosen;   // per chunk variant (may be null)
   int32_t int_cache; // integer CDF cache available (THREAD valid for any degree)
+  int32_t in_place;  // the chunks' neighbour lists are read over the bus
   MlpDev mlp;
   const void* mlp_w; // non-null when a model is loaded
 };

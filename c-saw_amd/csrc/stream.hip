@@ -58,7 +58,7 @@ __global__ void k_stream_degree(const int64_t* __restrict__ off, int64_t V, int3
 // picked neighbour, plus, for the first entry of each distinct vertex, the
 // neighbour list an integer CDF is built from.
 constexpr int64_t kBusRead = 64;
-constexpr int64_t kDirectFactor = 2;  // direct reads when 2x that estimate < the slice
+constexpr double kDirectFactor = 2.0;  // default: in place when 2x that estimate < the slice
 
 // partition id per frontier entry, per-partition counts and direct-read byte
 // estimates; for integer biases the vertices whose entries draw (d > 0,
@@ -193,6 +193,7 @@ __global__ __launch_bounds__(256) void k_part_scatter(HopDev H, const uint8_t* _
   }
 }
 
+constexpr int64_t kChunkedMin = 16384;  // frontier entries below which a partition is not binned
 constexpr int64_t kPieceBlocks = 1 << 17;  // packed blocks per staged piece (8M entries)
 
 // One wave per block of a staged piece: blocks pb0 + k (k < nb) at bytes
@@ -723,7 +724,7 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
         const bool held_p = held[0] == p || held[1] == p;
         if (o.slice_policy == CSAW_SLICE_DIRECT) direct[p] = 1;
         else if (o.slice_policy == CSAW_SLICE_AUTO && adj_bus && !held_p)
-          direct[p] = (double)pbytes[p] * kDirectFactor < slice_b;
+          direct[p] = (double)pbytes[p] * (o.direct_factor > 0.0f ? o.direct_factor : kDirectFactor) < slice_b;
       }
       // *out = a buffer, other than avoid, holding partition p (the copy is
       // issued when neither buffer holds it)
@@ -817,13 +818,16 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
           }
           last_use[buf] = ++use_tick;
         }
+        // small partitions are expanded one wave per entry whatever the
+        // schedule: binning and choosing cost more than they save there
+        const bool chunked = batched && (int64_t)cnt[p] >= (o.bin_min_entries > 0 ? o.bin_min_entries : kChunkedMin);
         hipEvent_t eb0, eb1, ex0, ex1;
         CSAW_TRY(event(&eb0));
         CSAW_TRY(event(&eb1));
         CSAW_TRY(event(&ex0));
         CSAW_TRY(event(&ex1));
         CSAW_HIP(hipEventRecord(eb0, cs));
-        if (batched) {
+        if (chunked) {
           // chunks of this partition's entries and their variants (needs only
           // degrees, so it runs while the slice is still in flight)
           CSAW_TRY(launch_fill(L, d_ctr, 0u, kCtrCount - 1));  // keeps the flags word
@@ -838,6 +842,7 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
           A.queue_cap = chunk_cap;
           A.chosen = nullptr;
           A.int_cache = int_bias ? 1 : 0;  // every drawing vertex has its CDF
+          A.in_place = direct[p] ? 1 : 0;
           A.mlp.w = d_mlp;
           A.mlp_w = d_mlp;
           const int64_t nblk_p = ((int64_t)cnt[p] + kBlockEntries - 1) / kBlockEntries;
@@ -897,7 +902,7 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
           }
           Gp.int_cdf[ck] = pool - a;  // global offsets index the slice-shaped array
         }
-        if (batched) {
+        if (chunked) {
           ExpandArgs X;
           X.queues = d_queues;
           X.queue_cap = chunk_cap;
@@ -912,7 +917,7 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
         CSAW_HIP(hipEventRecord(ex1, cs));
         if (buf >= 0) CSAW_HIP(hipEventRecord(freed[buf], cs));
         if (!seg_hub.empty()) CSAW_HIP(hipStreamSynchronize(cs));  // hub list buffers are reused
-        if (d_log) {
+        if (d_log && chunked) {
           // this partition's chunks and their timings, before the next reuses them
           CSAW_HIP(hipStreamSynchronize(cs));
           uint32_t nch = 0;
@@ -930,7 +935,7 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
             if (log_rows < o.log_cap) {
               if (o.log_features)
                 chunk_features(c.len, c.sum_d, c.max_d, c.sum_f, c.max_f, bucket_kind(c.bucket), c.hop, c.bucket, nch,
-                               o.log_features + log_rows * kMlpIn);
+                               o.log_features + log_rows * kMlpIn, direct[p]);
               if (o.log_variant) o.log_variant[log_rows] = lg[i].variant;
               if (o.log_log2_ticks) o.log_log2_ticks[log_rows] = std::log2((float)std::max<uint32_t>(lg[i].ticks, 1u));
               if (o.log_keys) o.log_keys[log_rows] = ((uint64_t)h << 40) | ((uint64_t)p << 32) | c.key;

@@ -334,6 +334,12 @@ typedef struct csaw_stream_opts {
   const struct csaw_tuning *tuning; /* launch shapes or NULL (defaults)      */
   int32_t slice_policy; /* csaw_slice_policy                                 */
   int32_t raw_slices;   /* copy unpacked slices even if the graph is packed  */
+  float direct_factor;  /* CSAW_SLICE_AUTO reads a partition in place when   */
+                        /*  factor x its estimated bus bytes < its slice's  */
+                        /*  bytes (0: 2)                                     */
+  int32_t bin_min_entries; /* batched: partitions with fewer frontier       */
+                        /*  entries run one wave per entry, unbinned (0:     */
+                        /*  16384; 1: bin every partition)                   */
   /* Per-chunk cost-model data (batched mode; each partition's chunks are
    * read back after it expands, so logging runs slower).  Rows as in
    * csaw_plan_chunk_log; keys are hop << 40 | partition << 32 | chunk key. */

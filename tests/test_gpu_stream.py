@@ -36,13 +36,16 @@ def test_stream_equals_in_memory(C, graphs, bias, parts):
 
 
 def _schedules(C):
+    # bin_min_entries=1: every partition is binned, however small
     yield "plain", dict(mode=C.MODE_PLAIN)
-    yield "heuristic", dict(mode=C.MODE_BATCHED, sched=C.SCHED_HEURISTIC)
+    yield "heuristic", dict(mode=C.MODE_BATCHED, sched=C.SCHED_HEURISTIC, bin_min_entries=1)
     for v in range(C.NUM_VARIANTS):
-        yield f"forced{v}", dict(mode=C.MODE_BATCHED, sched=C.SCHED_FORCED, variant=v)
-    yield "random", dict(mode=C.MODE_BATCHED, sched=C.SCHED_RANDOM, explore_key=5)
+        yield f"forced{v}", dict(mode=C.MODE_BATCHED, sched=C.SCHED_FORCED, variant=v, bin_min_entries=1)
+    yield "random", dict(mode=C.MODE_BATCHED, sched=C.SCHED_RANDOM, explore_key=5, bin_min_entries=1)
     # an untrained model picks arbitrary valid variants
-    yield "mlp", dict(mode=C.MODE_BATCHED, sched=C.SCHED_MLP, mlp=C.Mlp(3))
+    yield "mlp", dict(mode=C.MODE_BATCHED, sched=C.SCHED_MLP, mlp=C.Mlp(3), bin_min_entries=1)
+    # the default threshold: small partitions unbinned
+    yield "heuristic-default", dict(mode=C.MODE_BATCHED, sched=C.SCHED_HEURISTIC)
 
 
 @pytest.mark.parametrize("bias", [0, 1, 3])
@@ -59,7 +62,7 @@ def test_stream_schedules_equal_in_memory(C, graphs, bias):
             vc = list(st.variant_chunks)
             if name == "plain":
                 assert st.chunks == 0 and sum(vc) == 0
-            else:
+            elif name != "heuristic-default":
                 assert st.chunks > 0 and sum(vc) <= st.chunks
             if name == "forced0" and bias == 0:
                 # THREAD is valid for every uniform chunk with fanout <= 32
@@ -112,9 +115,9 @@ def test_stream_slice_policies(C, graphs, bias):
     want = C.sample(g, seeds, [15, 10, 5], bias=bias, request=4)
     parts = 7
     for pol in (C.SLICE_COPY, C.SLICE_DIRECT, C.SLICE_AUTO):
-        for name, kw in (("plain", dict(mode=C.MODE_PLAIN)), ("heuristic", dict()),
-                         ("forced0", dict(sched=C.SCHED_FORCED, variant=0)),
-                         ("forced3", dict(sched=C.SCHED_FORCED, variant=3))):
+        for name, kw in (("plain", dict(mode=C.MODE_PLAIN)), ("heuristic", dict(bin_min_entries=1)),
+                         ("forced0", dict(sched=C.SCHED_FORCED, variant=0, bin_min_entries=1)),
+                         ("forced3", dict(sched=C.SCHED_FORCED, variant=3, bin_min_entries=1))):
             got = C.stream_sample(hg, seeds, [15, 10, 5], bias=bias, request=4, partitions=parts,
                                   budget_bytes=1 << 30, slice_policy=pol, **kw)
             same(got, want, f"bias {bias} policy {pol} {name}")
@@ -183,7 +186,7 @@ def test_stream_chunk_log(C, graphs):
     seeds = g.seeds(800, request=5)
     for v in (0, 3):
         r = C.stream_sample(hg, seeds, [15, 10], bias=C.BIAS_DEGREE, partitions=4, budget_bytes=1 << 30,
-                            sched=C.SCHED_FORCED, variant=v, log_cap=1 << 16)
+                            sched=C.SCHED_FORCED, variant=v, log_cap=1 << 16, bin_min_entries=1)
         X, var, lt, keys = r["log"]
         st = r["stats"]
         assert 0 < len(keys) <= st.chunks
@@ -221,8 +224,8 @@ def test_stream_packed_slices(C, graphs, bias):
     seeds = g.seeds(1500, request=30 + bias)
     want = C.sample(g, seeds, [15, 10, 5], bias=bias, request=6)
     for raw in (False, True):
-        for name, kw in (("plain", dict(mode=C.MODE_PLAIN)), ("heuristic", dict()),
-                         ("forced1", dict(sched=C.SCHED_FORCED, variant=1))):
+        for name, kw in (("plain", dict(mode=C.MODE_PLAIN)), ("heuristic", dict(bin_min_entries=1)),
+                         ("forced1", dict(sched=C.SCHED_FORCED, variant=1, bin_min_entries=1))):
             got = C.stream_sample(hg, seeds, [15, 10, 5], bias=bias, request=6, partitions=5, budget_bytes=1 << 30,
                                   slice_policy=C.SLICE_COPY, raw_slices=raw, **kw)
             same(got, want, f"bias {bias} raw {raw} {name}")
