@@ -150,15 +150,25 @@ __global__ __launch_bounds__(kHubThreads) void k_hub_segments(GraphDev G, csaw_g
     const int64_t e = b + kHubSeg < d ? b + kHubSeg : d;
     const int64_t pbase = G.hub_pool[h];
     if (threadIdx.x == 0) carry_s = 0;
+    // the segment's neighbour ids, then their biases, all requested before
+    // any is used (16 rows of 256 entries): the list may be read in place
+    // over the bus, where every dependent round trip costs microseconds
+    constexpr int kRows = kHubSeg / kHubThreads;
+    int32_t u[kRows];
+    uint64_t w[kRows];
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      const int64_t i = b + r * kHubThreads + threadIdx.x;
+      u[r] = i < e ? G.adj[o + i] : -1;
+    }
+#pragma unroll
+    for (int r = 0; r < kRows; ++r)
+      w[r] = u[r] < 0 ? 0ull : (KIND == 0 ? (uint64_t)G.deg[u[r]] : (uint64_t)EdgeBias(u[r], &hv));
     __syncthreads();
-    for (int64_t i0 = b; i0 < e; i0 += kHubThreads) {
-      const int64_t i = i0 + threadIdx.x;
-      uint64_t wv = 0;
-      if (i < e) {
-        const int32_t u = G.adj[o + i];
-        wv = KIND == 0 ? (uint64_t)G.deg[u] : (uint64_t)EdgeBias(u, &hv);
-      }
-      uint64_t sc = wave_incl_scan_u64(wv);
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      const int64_t i = b + r * kHubThreads + threadIdx.x;
+      const uint64_t sc = wave_incl_scan_u64(w[r]);
       const int wid = threadIdx.x / kWave;
       if (lane_id() == kWave - 1) wsum[wid] = sc;
       __syncthreads();

@@ -148,16 +148,27 @@ __global__ __launch_bounds__(256) void k_frontier_cdf(GraphDev G, csaw_graph_vie
     const int64_t o = G.off[v];
     const int32_t d = (int32_t)(G.off[v + 1] - o);
     uint64_t acc = 0;
-    for (int i0 = 0; i0 < d; i0 += kWave) {
-      const int i = i0 + lane;
-      uint64_t w = 0;
-      if (i < d) {
-        const int32_t u = G.adj[o + i];
-        w = KIND == 0 ? (uint64_t)G.deg[u] : (uint64_t)EdgeBias(u, &hv);
+    // eight 64-entry rows at a time: their ids, then their biases, all
+    // requested before the scans (the list may be read over the bus)
+    constexpr int kRows = 8;
+    for (int i0 = 0; i0 < d; i0 += kRows * kWave) {
+      int32_t u[kRows];
+      uint64_t w[kRows];
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) {
+        const int i = i0 + r * kWave + lane;
+        u[r] = i < d ? G.adj[o + i] : -1;
       }
-      const uint64_t sc = wave_incl_scan_u64(w);
-      if (i < d) C[o - a + i] = acc + sc;
-      acc += readlane_u64(sc, kWave - 1);
+#pragma unroll
+      for (int r = 0; r < kRows; ++r)
+        w[r] = u[r] < 0 ? 0ull : (KIND == 0 ? (uint64_t)G.deg[u[r]] : (uint64_t)EdgeBias(u[r], &hv));
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) {
+        const int i = i0 + r * kWave + lane;
+        const uint64_t sc = wave_incl_scan_u64(w[r]);
+        if (i < d) C[o - a + i] = acc + sc;
+        acc += readlane_u64(sc, kWave - 1);
+      }
     }
   }
 }
