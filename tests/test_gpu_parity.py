@@ -212,14 +212,14 @@ def _cache_specs(seeds_of, fans, biases, dedups):
     return specs
 
 
-def _check_cache_plans(C, O, g, og, specs, what):
+def _check_cache_plans(C, O, g, og, specs, what, cdf_cache=True):
     """Every forced variant, the heuristic and random schedules, with the
     CDF cache (index searches, sub-wave groups) against the oracle."""
     want = [O.sample(og, s["seeds"], s["fanouts"], bias=s["bias"] if s["bias"] != 3 else O.BIAS_USER,
                      dedup=s["dedup"], request=s["request"]) for s in specs]
     runs = [(C.SCHED_FORCED, v) for v in range(C.NUM_VARIANTS)] + [(C.SCHED_HEURISTIC, 0), (C.SCHED_RANDOM, 0)]
     for sched, v in runs:
-        p = C.Plan(g, specs, mode=C.MODE_BATCHED, sched=sched, variant=v, cdf_cache=True, explore_key=5)
+        p = C.Plan(g, specs, mode=C.MODE_BATCHED, sched=sched, variant=v, cdf_cache=cdf_cache, explore_key=5)
         p.run()
         for i, s in enumerate(specs):
             same(p.result(i), want[i], f"{what} sched {sched} variant {v} req {i} {s['fanouts']} b{s['bias']}")
@@ -281,3 +281,19 @@ def test_mdrw_matches_oracle(C, O, g14):
     assert rc == 0
     n = off[-1]
     assert np.array_equal(off, oo) and np.array_equal(src[:n], so) and np.array_equal(dst[:n], do)
+
+
+@pytest.mark.parametrize("cdf_cache", [False, True])
+def test_limits_fanout_64_depth_8(C, O, g14, cdf_cache):
+    # the largest fanout (64, a whole wave per slot set) and the deepest
+    # request (8 hops), with every variant and the heuristic, beside an empty
+    # request and a single-seed one
+    g, og, *_ = g14
+    specs = [
+        {"seeds": g.seeds(40, request=0), "fanouts": [64, 3], "bias": 1, "dedup": 1, "request": 0},
+        {"seeds": g.seeds(20, request=1), "fanouts": [3, 2, 2, 2, 2, 2, 2, 2], "bias": 2, "dedup": 1, "request": 1},
+        {"seeds": g.seeds(0, request=2), "fanouts": [5, 5], "bias": 0, "dedup": 1, "request": 2},
+        {"seeds": g.seeds(1, request=3), "fanouts": [64, 64], "bias": 0, "dedup": 0, "request": 3},
+        {"seeds": g.seeds(30, request=4), "fanouts": [2, 64, 2], "bias": 3, "dedup": 1, "request": 4},
+    ]
+    _check_cache_plans(C, O, g, og, specs, "limits", cdf_cache=cdf_cache)
