@@ -479,10 +479,13 @@ __device__ __forceinline__ bool group_taken(int32_t ci, int32_t res, int jj, int
   return hit;
 }
 
-// Small-degree entries (d <= kGroupSmallCap) with 32-bit CDF keys: the group
-// copies the entry's CDF into its LDS slot with one coalesced load and every
-// draw is an upper_bound in LDS, with no further global access.
-constexpr int kGroupSmallCap = 64;
+// Small-degree entries with 32-bit CDF keys: the group copies the entry's CDF
+// into its LDS slot with coalesced loads and every draw is an upper_bound in
+// LDS, with no further global access.  The wave's LDS words are shared out
+// among its groups, so the slot (the largest degree taking this path) is
+// kGroupLdsWords / (64 / GS) words: 64 for groups of 4, 85 for fanout 5,
+// 170 for fanout 10, 256 for 16 and 512 for 32 lanes.
+constexpr int kGroupSmallCap = 64;  // slot words at the smallest group size
 
 template <int KIND>
 __device__ __forceinline__ int32_t gpick_small(const GEntry& en, const uint32_t* lc, const U4& r) {
@@ -495,17 +498,17 @@ __device__ __forceinline__ int32_t gpick_small(const GEntry& en, const uint32_t*
   }
 }
 
-// lc: the group's LDS slot of kGroupSmallCap words (null: no small path)
+// lc: the group's LDS slot of cap words (null: no small path)
 // gs: group size (4 .. 32, or 64), a run-time value so one build serves all;
 // the group is lanes gbase .. gbase + gs - 1 (gmask)
 template <int KIND, int KEYS>
 __device__ int32_t group_select(const GraphDev& G, const GEntry& en, int gs, int gl, int gbase, uint64_t gmask,
-                                uint32_t* lc) {
+                                uint32_t* lc, int cap) {
   const int F = en.F;
   const int32_t d = en.d;
   const bool act = gl < F;
   const uint32_t mode = (en.flags >> 4) & 15u;
-  const bool small = KIND != kUniform && lc && d <= kGroupSmallCap &&
+  const bool small = KIND != kUniform && lc && d <= cap &&
                      (mode == kPickInt32Idx || mode == kPickF32 || mode == kPickF32Idx);
   if (small) {
     const uint32_t* C = KIND == kF32 ? reinterpret_cast<const uint32_t*>(G.cdf)
@@ -602,7 +605,8 @@ __device__ void group_expand_batch(const GraphDev& G, const csaw_graph_view& hv,
   const int lane = lane_id();
   const int gi = lane / gs;  // group; gi == NG: the idle remainder lanes
   const int gl = lane - gi * gs, gbase = gi * gs;
-  uint32_t* lc = lds && gi < NG ? lds + gi * kGroupSmallCap : nullptr;
+  const int cap = kGroupLdsWords / NG;  // LDS slot words per group
+  uint32_t* lc = lds && gi < NG ? lds + gi * cap : nullptr;
   const uint64_t gmask = gs >= kWave ? ~0ull : (((1ull << gs) - 1ull) << gbase);
   GEntry mine;
   mine.d = 0;
@@ -616,11 +620,11 @@ __device__ void group_expand_batch(const GraphDev& G, const csaw_graph_view& hv,
     const uint32_t mode = (mine.flags >> 4) & 15u;
     if (KIND == kF32 && mode == kPickF32Idx)
       warm = *reinterpret_cast<const uint32_t*>(
-          mine.d <= kGroupSmallCap ? G.cdf + mine.o : cdf_index_first_block<float>(G.cdf, G.cdf_idx, G.E, mine.o, mine.d));
+          mine.d <= cap ? G.cdf + mine.o : cdf_index_first_block<float>(G.cdf, G.cdf_idx, G.E, mine.o, mine.d));
     if (KIND == kInt && mode == kPickInt32Idx) {
       const bool user = (mine.flags >> 1) & 1u;
       const uint32_t* C = user ? G.int_cdf32[1] : G.int_cdf32[0];
-      warm = mine.d <= kGroupSmallCap
+      warm = mine.d <= cap
                  ? C[mine.o]
                  : *cdf_index_first_block<uint32_t>(C, user ? G.int_idx32[1] : G.int_idx32[0], G.E, mine.o, mine.d);
     }
@@ -639,7 +643,7 @@ __device__ void group_expand_batch(const GraphDev& G, const csaw_graph_view& hv,
       if ((en.flags & 1u) && en.d <= en.F) {
         idx = gl < en.d ? gl : -1;
       } else {
-        const int32_t res = group_select<KIND, KEYS>(G, en, gs, gl, gbase, gmask, lc);
+        const int32_t res = group_select<KIND, KEYS>(G, en, gs, gl, gbase, gmask, lc, cap);
         idx = gl < en.F ? res : -1;
       }
     }
