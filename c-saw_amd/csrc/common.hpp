@@ -178,6 +178,9 @@ __host__ __device__ __forceinline__ int entry_bucket(int64_t d, int32_t F, int32
   if (dedup && d <= F) return kBucketCopy;
   int kind = bias_kind(bias);
   int l = ceil_log2_u32((uint32_t)(d > 0x7fffffff ? 0x7fffffff : d));
+  // a uniform draw costs the same at any degree: degrees up to 1024 share
+  // one class, so a block's uniform entries form few, full chunks
+  if (kind == kUniform && l <= 10) l = 10;
   return 1 + kDegClasses * kind + degree_class(l);
 }
 __host__ __device__ __forceinline__ int bucket_kind(int bucket) { return (bucket - 1) / kDegClasses; }

@@ -28,7 +28,7 @@ template <bool BIN, bool LIST = false>
 __global__ __launch_bounds__(256) void k_count(GraphDev G, const ReqDev* __restrict__ R, const int32_t* __restrict__ gs2req,
                                                HopDev H, const int32_t* __restrict__ list = nullptr,
                                                int64_t n_list = 0) {
-  __shared__ uint32_t wsum[4];
+  __shared__ uint32_t wsum[4][4];  // [item row][wave] totals of the block scan
   __shared__ uint32_t hist[kNumBuckets];
   __shared__ uint32_t bstart[kNumBuckets];
   __shared__ uint32_t fsum[kNumBuckets];
@@ -49,15 +49,18 @@ __global__ __launch_bounds__(256) void k_count(GraphDev G, const ReqDev* __restr
       }
       __syncthreads();
     }
+    // item i of thread tid is block position i * 256 + tid: a wave's 64 lanes
+    // hold 64 consecutive entries, so the bucket slots one wave instruction
+    // hands out keep runs of neighbouring entries (siblings, whose edges
+    // are neighbours in the output) together in a chunk
     uint32_t k[4];
     int bk[4];
     uint32_t slot[4];
     int32_t eid[4];
-    uint32_t tsum = 0;
     unsigned long long tbytes = 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int64_t pos = base + tid * 4 + i;
+      const int64_t pos = base + i * 256 + tid;
       k[i] = 0;
       bk[i] = -1;
       eid[i] = 0;
@@ -87,26 +90,31 @@ __global__ __launch_bounds__(256) void k_count(GraphDev G, const ReqDev* __restr
           atomicAdd(&lsum[bk[i]], (uint32_t)ceil_log2_u32((uint32_t)d));
         }
       }
-      tsum += k[i];
     }
     if (!BIN && H.bytes) {
       for (int off = 32; off >= 1; off >>= 1) tbytes += __shfl_xor(tbytes, off, 64);
       if (lane == 0 && tbytes) atomicAdd(&H.bytes[kKernPlain], tbytes);
     }
     if (!LIST) {
-      // block exclusive scan of per-thread sums
-      const uint32_t incl = wave_incl_scan_u32(tsum);
-      if (lane == kWave - 1) wsum[wid] = incl;
-      __syncthreads();
-      uint32_t pre = incl - tsum;
-      for (int w = 0; w < wid; ++w) pre += wsum[w];
+      // block exclusive scan of the per-entry counts in position order:
+      // row i (positions i * 256 .. i * 256 + 255) after rows 0 .. i-1
+      uint32_t incl[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int64_t e = base + tid * 4 + i;
-        if (e < n) H.local_excl[e] = pre;
-        pre += k[i];
+        incl[i] = wave_incl_scan_u32(k[i]);
+        if (lane == kWave - 1) wsum[i][wid] = incl[i];
       }
-      if (tid == 255) H.blk_excl[blk] = pre;  // block total, scanned later
+      __syncthreads();
+      uint32_t row = 0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        uint32_t pre = row + incl[i] - k[i];
+        for (int w = 0; w < wid; ++w) pre += wsum[i][w];
+        const int64_t e = base + i * 256 + tid;
+        if (e < n) H.local_excl[e] = pre;
+        row += wsum[i][0] + wsum[i][1] + wsum[i][2] + wsum[i][3];
+      }
+      if (tid == 0) H.blk_excl[blk] = row;  // block total, scanned later
     } else {
       __syncthreads();  // the histogram is complete
     }
@@ -160,8 +168,8 @@ __global__ __launch_bounds__(256) void k_count(GraphDev G, const ReqDev* __restr
           }
         }
       }
-      __syncthreads();
     }
+    __syncthreads();  // every wave is done with wsum and the histograms before the next block
   }
 }
 

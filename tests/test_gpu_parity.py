@@ -268,6 +268,24 @@ def test_walk_matches_oracle(C, O, g14, bias):
     assert np.array_equal(W, Wo) and steps == so
 
 
+def test_walk_more_walkers_than_lanes(C, O, g14):
+    # 600,000 walkers outnumber the lanes of one launch (256 CUs x 8 blocks
+    # x 256), so blocks take a second slice of walkers; a length of 37
+    # leaves a partial tile of staged steps; lengths 0 and 1 and no walkers
+    g, og, *_ = g14
+    seeds = g.seeds(600000, request=6)
+    W, steps = C.random_walk(g, seeds, 37, bias=2)
+    Wo, so = O.walk(og, seeds, 37, bias=2, nthreads=8)
+    assert np.array_equal(W, Wo) and steps == so
+    assert (W[:, 1:] < 0).any() and (W[:, -1] >= 0).any()  # both ended and full walks
+    for length in (0, 1):
+        W, steps = C.random_walk(g, seeds[:5000], length, bias=1)
+        Wo, so = O.walk(og, seeds[:5000], length, bias=1)
+        assert np.array_equal(W, Wo) and steps == so
+    W, steps = C.random_walk(g, seeds[:0], 10, bias=0)
+    assert W.shape == (0, 11) and steps == 0
+
+
 def test_mdrw_matches_oracle(C, O, g14):
     g, og, *_ = g14
     S, FS, D = 64, 16, 50
