@@ -485,6 +485,29 @@ static int build_index(const T* C, int64_t E, T** out, hipStream_t stream, int n
   return CSAW_OK;
 }
 
+__global__ void k_walk_table(const int64_t* __restrict__ off, const float* __restrict__ cdf, int64_t V,
+                             WalkVertex* __restrict__ out) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t o = off[v];
+    const int64_t d = off[v + 1] - o;
+    WalkVertex r;
+    r.off = o;
+    r.total = d > 0 ? cdf[o + d - 1] : 0.0f;
+    r.deg = (int32_t)d;
+    out[v] = r;
+  }
+}
+
+int graph_prepare_walk_table(csaw_graph* g) {
+  CSAW_TRY(graph_prepare_bias(g, CSAW_BIAS_WEIGHT));
+  if (g->walk_vt || g->V == 0) return CSAW_OK;
+  CSAW_HIP(hipMalloc(&g->walk_vt, sizeof(WalkVertex) * g->V));
+  k_walk_table<<<grid_for(g->V, 256, g->num_cu), 256, 0, g->stream>>>(g->off, g->cdf, g->V, g->walk_vt);
+  CSAW_LAUNCH_CHECK();
+  CSAW_HIP(hipStreamSynchronize(g->stream));
+  return CSAW_OK;
+}
+
 int graph_prepare_index(csaw_graph* g, int32_t bias) {
   if (bias == CSAW_BIAS_WEIGHT) {
     CSAW_TRY(graph_prepare_bias(g, bias));
@@ -565,6 +588,7 @@ static void graph_free(csaw_graph* g) {
     (void)hipFree(g->int_idx32[k]);
   }
   (void)hipFree(g->cdf_idx);
+  (void)hipFree(g->walk_vt);
   if (g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
 }

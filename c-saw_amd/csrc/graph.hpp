@@ -25,6 +25,8 @@ struct csaw_graph {
   uint32_t* int_cdf32[2] = {nullptr, nullptr};
   uint32_t* int_idx32[2] = {nullptr, nullptr};
   int64_t hub_entries = 0;
+  // walk vertex table (built for FP32 walks): offset, CDF total, degree
+  struct WalkVertex* walk_vt = nullptr;
   hipStream_t stream = nullptr;
   int device = 0;
   int num_cu = 0;
@@ -33,7 +35,18 @@ struct csaw_graph {
   csaw_graph_view hook_view() const;
 };
 
+// One 16-byte record per vertex for FP32 walks: a step reads offset, degree
+// and the CDF total C[off + deg - 1] with one aligned load instead of one
+// for the offsets and another for the total.
+struct alignas(16) WalkVertex {
+  int64_t off;
+  float total;  // C[off + deg - 1] of the FP32 CDF, 0 when deg == 0
+  int32_t deg;
+};
+
 namespace csaw {
+// Builds the walk vertex table from the FP32 CDF (built first when missing).
+int graph_prepare_walk_table(csaw_graph* g);
 // Builds lazily what a bias kind needs: FP32 CDF (CSAW_BIAS_WEIGHT) or the
 // integer hub cache (CSAW_BIAS_DEGREE / CSAW_BIAS_USER).
 int graph_prepare_bias(csaw_graph* g, int32_t bias);

@@ -17,19 +17,21 @@ namespace csaw {
 constexpr int kWalkTile = 16;
 constexpr int kWalkThreads = 256;
 
+// FP32 pick given the vertex's CDF total T = C[o + d - 1].
+__device__ __forceinline__ int32_t walk_pick_f32(const GraphDev& G, int64_t o, int32_t d, float T, const U4& r) {
+  if (T > 0.0f) {
+    const float x = __fmul_rn(philox_u24(r), T);
+    if (G.cdf_idx) return cdf_index_search<float>(G.cdf, G.cdf_idx, G.E, o, d, x);
+    return (int32_t)upper_bound_dev<float>(G.cdf + o, d, x);
+  }
+  return (int32_t)mulhi64(philox_t64(r), (uint64_t)d);
+}
+
 template <int KIND>
 __device__ __forceinline__ int32_t walk_pick(const GraphDev& G, const csaw_graph_view& hv, int32_t bias, int32_t v,
                                              int64_t o, int32_t d, const U4& r) {
   const uint64_t t = philox_t64(r);
-  if (KIND == kF32) {
-    const float T = G.cdf[o + d - 1];
-    if (T > 0.0f) {
-      const float x = __fmul_rn(philox_u24(r), T);
-      if (G.cdf_idx) return cdf_index_search<float>(G.cdf, G.cdf_idx, G.E, o, d, x);
-      return (int32_t)upper_bound_dev<float>(G.cdf + o, d, x);
-    }
-    return (int32_t)mulhi64(t, (uint64_t)d);
-  }
+  if (KIND == kF32) return walk_pick_f32(G, o, d, G.cdf[o + d - 1], r);
   if (KIND == kInt) {
     const int ck = bias == CSAW_BIAS_USER ? 1 : 0;
     if (G.int_cdf32[ck] && G.int_idx32[ck]) {
@@ -69,7 +71,8 @@ template <int KIND>
 __global__ __launch_bounds__(kWalkThreads) void k_walk(GraphDev G, csaw_graph_view hv, int32_t bias, uint32_t k0,
                                                        uint32_t k1, uint32_t request, const int32_t* __restrict__ seeds,
                                                        int64_t n, int32_t len, int32_t* __restrict__ W,
-                                                       unsigned long long* __restrict__ steps_out) {
+                                                       unsigned long long* __restrict__ steps_out,
+                                                       const WalkVertex* __restrict__ vt) {
   __shared__ int32_t tile[kWalkTile][kWalkThreads + 1];
   const uint32_t c3 = (kDomWalk << 28) | request;
   unsigned long long moved = 0, bytes = 0;
@@ -84,13 +87,25 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk(GraphDev G, csaw_graph_vi
       for (int32_t k = 0; k < tn; ++k) {
         const int32_t t = t0 + k;
         if (act && v >= 0) {
-          const int64_t o = G.off[v];
-          const int32_t d = (int32_t)(G.off[v + 1] - o);  // same line as off[v]
+          int64_t o;
+          int32_t d;
+          float T = 0.0f;
+          if (KIND == kF32 && vt) {
+            const WalkVertex rec = vt[v];  // offset, degree and CDF total in one 16-byte load
+            o = rec.off;
+            d = rec.deg;
+            T = rec.total;
+          } else {
+            o = G.off[v];
+            d = (int32_t)(G.off[v + 1] - o);  // same line as off[v]
+          }
           if (d == 0) {
             v = -1;
           } else {
             const U4 r = philox4x32_10((uint32_t)t, (uint32_t)w, (uint32_t)(w >> 32), c3, k0, k1);
-            v = G.adj[o + walk_pick<KIND>(G, hv, bias, v, o, d, r)];
+            const int32_t pick =
+                KIND == kF32 && vt ? walk_pick_f32(G, o, d, T, r) : walk_pick<KIND>(G, hv, bias, v, o, d, r);
+            v = G.adj[o + pick];
             ++moved;
             // algorithmic bytes of the step: offset 8 + degree 4 + picked id 4 +
             // walk entry 4, plus the CDF total and ceil(log2 d) probes (FP32 /
@@ -123,19 +138,19 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk(GraphDev G, csaw_graph_vi
 
 int launch_walk(hipStream_t st, int num_cu, int per_cu, const GraphDev& G, const csaw_graph_view& hv, int32_t bias,
                 uint64_t key, uint32_t request, const int32_t* seeds, int64_t n, int32_t len, int32_t* W,
-                unsigned long long* steps) {
+                unsigned long long* steps, const WalkVertex* vt) {
   const int64_t need = (n + kWalkThreads - 1) / kWalkThreads;
   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)num_cu * per_cu));
   const int kind = bias_kind(bias);
   if (kind == kUniform)
     k_walk<kUniform><<<grid, kWalkThreads, 0, st>>>(G, hv, bias, (uint32_t)key, (uint32_t)(key >> 32), request, seeds, n,
-                                                    len, W, steps);
+                                                    len, W, steps, nullptr);
   else if (kind == kInt)
     k_walk<kInt><<<grid, kWalkThreads, 0, st>>>(G, hv, bias, (uint32_t)key, (uint32_t)(key >> 32), request, seeds, n, len,
-                                                W, steps);
+                                                W, steps, nullptr);
   else
     k_walk<kF32><<<grid, kWalkThreads, 0, st>>>(G, hv, bias, (uint32_t)key, (uint32_t)(key >> 32), request, seeds, n, len,
-                                                W, steps);
+                                                W, steps, vt);
   CSAW_LAUNCH_CHECK();
   return CSAW_OK;
 }
@@ -170,6 +185,7 @@ extern "C" int csaw_walk_plan_create(csaw_graph* g, int32_t bias, uint64_t seed,
     if (seeds[i] < 0 || seeds[i] >= g->V) return CSAW_ERR_INVALID_ARG;
   // walks search the CDF cache (FP32 CDF or integer cache) through its index
   CSAW_TRY(graph_prepare_index(g, bias));
+  if (bias == CSAW_BIAS_WEIGHT) CSAW_TRY(graph_prepare_walk_table(g));
   csaw_walk_plan* p = new csaw_walk_plan();
   p->g = g;
   p->bias = bias;
@@ -207,7 +223,8 @@ extern "C" int csaw_walk_plan_run(csaw_walk_plan* p, void* stream, int sync) {
   CSAW_HIP(hipMemsetAsync(p->d_steps, 0, 2 * sizeof(unsigned long long), st));
   CSAW_HIP(hipEventRecord(e0, st));
   CSAW_TRY(launch_walk(st, p->g->num_cu, p->per_cu, p->g->view(true), p->g->hook_view(), p->bias, p->key, p->request,
-                       p->d_seeds, p->n, p->len, p->d_W, p->d_steps));
+                       p->d_seeds, p->n, p->len, p->d_W, p->d_steps,
+                       p->bias == CSAW_BIAS_WEIGHT ? p->g->walk_vt : nullptr));
   CSAW_HIP(hipEventRecord(e1, st));
   if (sync) {
     CSAW_HIP(hipEventSynchronize(e1));
