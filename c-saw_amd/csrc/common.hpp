@@ -59,6 +59,15 @@ constexpr int kChunkMin = 8;         // smallest chunk piece (buckets split for 
 enum BiasKind : int { kUniform = 0, kInt = 1, kF32 = 2 };
 
 // ----------------------------------------------------------- device views
+// One 16-byte record per vertex for a cached CDF: offset, degree and the
+// CDF total C[off + deg - 1] (FP32 bits or uint32) in one aligned load,
+// instead of the offset pair and then the total.
+struct alignas(16) VertexRecord {
+  int64_t off;
+  uint32_t total;  // 0 when deg == 0
+  int32_t deg;
+};
+
 struct GraphDev {
   const int64_t* off;
   const int32_t* adj;
@@ -73,6 +82,9 @@ struct GraphDev {
   const uint64_t* int_idx[2];   // 16-ary search indexes over int_cdf
   const uint32_t* int_cdf32[2]; // the cache as uint32 when all totals fit (int_cdf null then)
   const uint32_t* int_idx32[2];
+  // [DEGREE, USER, WEIGHT] vertex records over the uint32 integer caches and
+  // the FP32 CDF (null when that cache is not in the view)
+  const VertexRecord* vrec[3] = {nullptr, nullptr, nullptr};
   int32_t n_hubs;
   int64_t V, E;
 };

@@ -485,24 +485,33 @@ static int build_index(const T* C, int64_t E, T** out, hipStream_t stream, int n
   return CSAW_OK;
 }
 
-__global__ void k_walk_table(const int64_t* __restrict__ off, const float* __restrict__ cdf, int64_t V,
-                             WalkVertex* __restrict__ out) {
+template <typename T>
+__global__ void k_vertex_records(const int64_t* __restrict__ off, const T* __restrict__ C, int64_t V,
+                                 VertexRecord* __restrict__ out) {
   for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
     const int64_t o = off[v];
     const int64_t d = off[v + 1] - o;
-    WalkVertex r;
+    VertexRecord r;
     r.off = o;
-    r.total = d > 0 ? cdf[o + d - 1] : 0.0f;
+    uint32_t t = 0;
+    if (d > 0) memcpy(&t, &C[o + d - 1], 4);
+    r.total = t;
     r.deg = (int32_t)d;
     out[v] = r;
   }
 }
 
-int graph_prepare_walk_table(csaw_graph* g) {
-  CSAW_TRY(graph_prepare_bias(g, CSAW_BIAS_WEIGHT));
-  if (g->walk_vt || g->V == 0) return CSAW_OK;
-  CSAW_HIP(hipMalloc(&g->walk_vt, sizeof(WalkVertex) * g->V));
-  k_walk_table<<<grid_for(g->V, 256, g->num_cu), 256, 0, g->stream>>>(g->off, g->cdf, g->V, g->walk_vt);
+int graph_prepare_vertex_records(csaw_graph* g, int32_t bias) {
+  const int k = bias == CSAW_BIAS_DEGREE ? 0 : (bias == CSAW_BIAS_USER ? 1 : (bias == CSAW_BIAS_WEIGHT ? 2 : -1));
+  if (k < 0 || g->vrec[k] || g->V == 0) return CSAW_OK;
+  if (k == 2 && !g->cdf) return CSAW_OK;
+  if (k < 2 && !g->int_cdf32[k]) return CSAW_OK;
+  CSAW_HIP(hipMalloc(&g->vrec[k], sizeof(VertexRecord) * g->V));
+  const int grid = grid_for(g->V, 256, g->num_cu);
+  if (k == 2)
+    k_vertex_records<float><<<grid, 256, 0, g->stream>>>(g->off, g->cdf, g->V, g->vrec[k]);
+  else
+    k_vertex_records<uint32_t><<<grid, 256, 0, g->stream>>>(g->off, g->int_cdf32[k], g->V, g->vrec[k]);
   CSAW_LAUNCH_CHECK();
   CSAW_HIP(hipStreamSynchronize(g->stream));
   return CSAW_OK;
@@ -551,6 +560,7 @@ csaw::GraphDev csaw_graph::view(bool use_cache) const {
     G.int_idx32[k] = use_cache ? int_idx32[k] : nullptr;
   }
   G.cdf_idx = use_cache ? cdf_idx : nullptr;
+  for (int k = 0; k < 3; ++k) G.vrec[k] = use_cache ? vrec[k] : nullptr;
   G.n_hubs = n_hubs < 0 ? 0 : n_hubs;
   G.V = V;
   G.E = E;
@@ -588,7 +598,7 @@ static void graph_free(csaw_graph* g) {
     (void)hipFree(g->int_idx32[k]);
   }
   (void)hipFree(g->cdf_idx);
-  (void)hipFree(g->walk_vt);
+  for (int k = 0; k < 3; ++k) (void)hipFree(g->vrec[k]);
   if (g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
 }

@@ -25,8 +25,9 @@ struct csaw_graph {
   uint32_t* int_cdf32[2] = {nullptr, nullptr};
   uint32_t* int_idx32[2] = {nullptr, nullptr};
   int64_t hub_entries = 0;
-  // walk vertex table (built for FP32 walks): offset, CDF total, degree
-  struct WalkVertex* walk_vt = nullptr;
+  // vertex records (offset, CDF total, degree) over the cached CDFs:
+  // [DEGREE, USER] from the uint32 integer caches, [WEIGHT] from the FP32 CDF
+  csaw::VertexRecord* vrec[3] = {nullptr, nullptr, nullptr};
   hipStream_t stream = nullptr;
   int device = 0;
   int num_cu = 0;
@@ -35,18 +36,10 @@ struct csaw_graph {
   csaw_graph_view hook_view() const;
 };
 
-// One 16-byte record per vertex for FP32 walks: a step reads offset, degree
-// and the CDF total C[off + deg - 1] with one aligned load instead of one
-// for the offsets and another for the total.
-struct alignas(16) WalkVertex {
-  int64_t off;
-  float total;  // C[off + deg - 1] of the FP32 CDF, 0 when deg == 0
-  int32_t deg;
-};
-
 namespace csaw {
-// Builds the walk vertex table from the FP32 CDF (built first when missing).
-int graph_prepare_walk_table(csaw_graph* g);
+// Builds the vertex records of a bias's cached CDF (after graph_prepare_index;
+// nothing for uniform, or for an integer cache kept as uint64).
+int graph_prepare_vertex_records(csaw_graph* g, int32_t bias);
 // Builds lazily what a bias kind needs: FP32 CDF (CSAW_BIAS_WEIGHT) or the
 // integer hub cache (CSAW_BIAS_DEGREE / CSAW_BIAS_USER).
 int graph_prepare_bias(csaw_graph* g, int32_t bias);

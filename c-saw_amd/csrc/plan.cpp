@@ -358,7 +358,10 @@ extern "C" int csaw_plan_create(csaw_graph* g, const csaw_request* reqs, int32_t
   auto body = [&]() -> int {
     for (int i = 0; i < n_req; ++i) {
       CSAW_TRY(graph_prepare_bias(g, reqs[i].bias));
-      if (o.cdf_cache) CSAW_TRY(graph_prepare_index(g, reqs[i].bias));
+      if (o.cdf_cache) {
+        CSAW_TRY(graph_prepare_index(g, reqs[i].bias));
+        CSAW_TRY(graph_prepare_vertex_records(g, reqs[i].bias));
+      }
       p->reqs.push_back(reqs[i]);
       p->seeds.emplace_back(reqs[i].seeds, reqs[i].seeds + reqs[i].samples);
       p->reqs.back().seeds = nullptr;

@@ -30,6 +30,8 @@ struct Entry {
   int32_t g;      // global sample id (written to the sample column)
   int64_t pos;    // first output slot
   uint32_t c1, c2, c3, k0, k1;
+  uint32_t T;     // CDF total from the vertex record (has_T)
+  bool has_T;
 };
 
 __device__ __forceinline__ Entry load_entry(const GraphDev& G, const ReqDev* __restrict__ R,
@@ -41,9 +43,21 @@ __device__ __forceinline__ Entry load_entry(const GraphDev& G, const ReqDev* __r
   en.F = r.fanout[H.hop];
   en.dedup = r.dedup;
   en.bias = r.bias;
-  // offset and degree from one pair of offsets (one line, not two)
-  en.o = G.off[en.v];
-  en.d = (int32_t)(G.off[en.v + 1] - en.o);
+  // offset, degree and CDF total from the bias's vertex record when the view
+  // has one, else offset and degree from one pair of offsets (one line)
+  const int rk = en.bias == CSAW_BIAS_DEGREE ? 0 : (en.bias == CSAW_BIAS_USER ? 1 : 2);
+  const VertexRecord* vr = en.bias == CSAW_BIAS_UNIFORM ? nullptr : G.vrec[rk];
+  en.has_T = vr != nullptr;
+  en.T = 0;
+  if (vr) {
+    const VertexRecord rec = vr[en.v];
+    en.o = rec.off;
+    en.d = rec.deg;
+    en.T = rec.total;
+  } else {
+    en.o = G.off[en.v];
+    en.d = (int32_t)(G.off[en.v + 1] - en.o);
+  }
   en.pos = (int64_t)H.grp_excl[e >> kGroupShift] + H.blk_excl[e >> kBlockShift] + H.local_excl[e];
   en.c1 = (uint32_t)(e - H.fo[en.g]);
   en.c2 = (uint32_t)(en.g - r.gbase);
@@ -117,7 +131,7 @@ __device__ __forceinline__ bool picker_global(const GraphDev& G, const Entry& en
   if (kind == kF32) {
     pk.C = G.cdf;
     pk.K = G.cdf_idx;
-    pk.Tf = G.cdf[en.o + en.d - 1];
+    pk.Tf = en.has_T ? __uint_as_float(en.T) : G.cdf[en.o + en.d - 1];
     if (pk.Tf > 0.0f) pk.mode = G.cdf_idx ? kPickF32Idx : kPickF32;
     return true;
   }
@@ -125,7 +139,7 @@ __device__ __forceinline__ bool picker_global(const GraphDev& G, const Entry& en
   if (G.int_cdf32[ck]) {
     pk.C = G.int_cdf32[ck];
     pk.K = G.int_idx32[ck];
-    pk.T = G.int_cdf32[ck][en.o + en.d - 1];
+    pk.T = en.has_T ? en.T : G.int_cdf32[ck][en.o + en.d - 1];
     if (pk.T > 0) pk.mode = kPickInt32Idx;
     return true;
   }
@@ -304,6 +318,8 @@ __device__ __forceinline__ Entry bcast_entry(const Entry& m, int j) {
   e.c3 = readlane_u(m.c3, j);
   e.k0 = readlane_u(m.k0, j);
   e.k1 = readlane_u(m.k1, j);
+  e.T = readlane_u(m.T, j);
+  e.has_T = readlane((int32_t)m.has_T, j) != 0;
   return e;
 }
 
@@ -399,12 +415,12 @@ __device__ __forceinline__ GEntry make_gentry(const GraphDev& G, const Entry& en
   ge.T = 0;
   if (en.d > 0 && en.F > 0 && !(en.dedup && en.d <= en.F)) {
     if (KIND == kF32) {
-      const float Tf = G.cdf[en.o + en.d - 1];
+      const float Tf = en.has_T ? __uint_as_float(en.T) : G.cdf[en.o + en.d - 1];
       ge.T = __float_as_uint(Tf);
       if (Tf > 0.0f) mode = G.cdf_idx ? kPickF32Idx : kPickF32;
     } else if (KIND == kInt) {
       if (G.int_cdf32[ck]) {
-        ge.T = G.int_cdf32[ck][en.o + en.d - 1];
+        ge.T = en.has_T ? en.T : G.int_cdf32[ck][en.o + en.d - 1];
         if (ge.T > 0) mode = kPickInt32Idx;
       } else if (G.int_cdf[ck]) {
         ge.T = G.int_cdf[ck][en.o + en.d - 1];

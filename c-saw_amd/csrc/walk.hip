@@ -71,10 +71,10 @@ template <int KIND>
 __global__ __launch_bounds__(kWalkThreads) void k_walk(GraphDev G, csaw_graph_view hv, int32_t bias, uint32_t k0,
                                                        uint32_t k1, uint32_t request, const int32_t* __restrict__ seeds,
                                                        int64_t n, int32_t len, int32_t* __restrict__ W,
-                                                       unsigned long long* __restrict__ steps_out,
-                                                       const WalkVertex* __restrict__ vt) {
+                                                       unsigned long long* __restrict__ steps_out) {
   __shared__ int32_t tile[kWalkTile][kWalkThreads + 1];
   const uint32_t c3 = (kDomWalk << 28) | request;
+  const VertexRecord* __restrict__ vt = KIND == kF32 ? G.vrec[2] : nullptr;
   unsigned long long moved = 0, bytes = 0;
   for (int64_t w0 = (int64_t)blockIdx.x * kWalkThreads; w0 < n; w0 += (int64_t)gridDim.x * kWalkThreads) {
     const int64_t w = w0 + threadIdx.x;
@@ -91,10 +91,10 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk(GraphDev G, csaw_graph_vi
           int32_t d;
           float T = 0.0f;
           if (KIND == kF32 && vt) {
-            const WalkVertex rec = vt[v];  // offset, degree and CDF total in one 16-byte load
+            const VertexRecord rec = vt[v];  // offset, degree and CDF total in one 16-byte load
             o = rec.off;
             d = rec.deg;
-            T = rec.total;
+            T = __uint_as_float(rec.total);
           } else {
             o = G.off[v];
             d = (int32_t)(G.off[v + 1] - o);  // same line as off[v]
@@ -138,19 +138,19 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk(GraphDev G, csaw_graph_vi
 
 int launch_walk(hipStream_t st, int num_cu, int per_cu, const GraphDev& G, const csaw_graph_view& hv, int32_t bias,
                 uint64_t key, uint32_t request, const int32_t* seeds, int64_t n, int32_t len, int32_t* W,
-                unsigned long long* steps, const WalkVertex* vt) {
+                unsigned long long* steps) {
   const int64_t need = (n + kWalkThreads - 1) / kWalkThreads;
   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)num_cu * per_cu));
   const int kind = bias_kind(bias);
   if (kind == kUniform)
     k_walk<kUniform><<<grid, kWalkThreads, 0, st>>>(G, hv, bias, (uint32_t)key, (uint32_t)(key >> 32), request, seeds, n,
-                                                    len, W, steps, nullptr);
+                                                    len, W, steps);
   else if (kind == kInt)
     k_walk<kInt><<<grid, kWalkThreads, 0, st>>>(G, hv, bias, (uint32_t)key, (uint32_t)(key >> 32), request, seeds, n, len,
-                                                W, steps, nullptr);
+                                                W, steps);
   else
     k_walk<kF32><<<grid, kWalkThreads, 0, st>>>(G, hv, bias, (uint32_t)key, (uint32_t)(key >> 32), request, seeds, n, len,
-                                                W, steps, vt);
+                                                W, steps);
   CSAW_LAUNCH_CHECK();
   return CSAW_OK;
 }
@@ -185,7 +185,7 @@ extern "C" int csaw_walk_plan_create(csaw_graph* g, int32_t bias, uint64_t seed,
     if (seeds[i] < 0 || seeds[i] >= g->V) return CSAW_ERR_INVALID_ARG;
   // walks search the CDF cache (FP32 CDF or integer cache) through its index
   CSAW_TRY(graph_prepare_index(g, bias));
-  if (bias == CSAW_BIAS_WEIGHT) CSAW_TRY(graph_prepare_walk_table(g));
+  if (bias == CSAW_BIAS_WEIGHT) CSAW_TRY(graph_prepare_vertex_records(g, bias));
   csaw_walk_plan* p = new csaw_walk_plan();
   p->g = g;
   p->bias = bias;
@@ -223,8 +223,7 @@ extern "C" int csaw_walk_plan_run(csaw_walk_plan* p, void* stream, int sync) {
   CSAW_HIP(hipMemsetAsync(p->d_steps, 0, 2 * sizeof(unsigned long long), st));
   CSAW_HIP(hipEventRecord(e0, st));
   CSAW_TRY(launch_walk(st, p->g->num_cu, p->per_cu, p->g->view(true), p->g->hook_view(), p->bias, p->key, p->request,
-                       p->d_seeds, p->n, p->len, p->d_W, p->d_steps,
-                       p->bias == CSAW_BIAS_WEIGHT ? p->g->walk_vt : nullptr));
+                       p->d_seeds, p->n, p->len, p->d_W, p->d_steps));
   CSAW_HIP(hipEventRecord(e1, st));
   if (sync) {
     CSAW_HIP(hipEventSynchronize(e1));
