@@ -27,6 +27,14 @@ __device__ __forceinline__ int32_t walk_pick_f32(const GraphDev& G, int64_t o, i
   return (int32_t)mulhi64(philox_t64(r), (uint64_t)d);
 }
 
+// Integer pick over a uint32 CDF cache given its total T = C[o + d - 1].
+__device__ __forceinline__ int32_t walk_pick_i32(const GraphDev& G, int ck, int64_t o, int32_t d, uint32_t T,
+                                                 const U4& r) {
+  const uint64_t t = philox_t64(r);
+  if (T > 0) return cdf_index_search<uint32_t>(G.int_cdf32[ck], G.int_idx32[ck], G.E, o, d, (uint32_t)mulhi64(t, T));
+  return (int32_t)mulhi64(t, (uint64_t)d);
+}
+
 template <int KIND>
 __device__ __forceinline__ int32_t walk_pick(const GraphDev& G, const csaw_graph_view& hv, int32_t bias, int32_t v,
                                              int64_t o, int32_t d, const U4& r) {
@@ -74,7 +82,10 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk(GraphDev G, csaw_graph_vi
                                                        unsigned long long* __restrict__ steps_out) {
   __shared__ int32_t tile[kWalkTile][kWalkThreads + 1];
   const uint32_t c3 = (kDomWalk << 28) | request;
-  const VertexRecord* __restrict__ vt = KIND == kF32 ? G.vrec[2] : nullptr;
+  // vertex records of the bias's cache (offset, degree, CDF total in one load)
+  const int ck = bias == CSAW_BIAS_USER ? 1 : 0;
+  const VertexRecord* __restrict__ vt =
+      KIND == kF32 ? G.vrec[2] : (KIND == kInt && G.int_cdf32[ck] && G.int_idx32[ck] ? G.vrec[ck] : nullptr);
   unsigned long long moved = 0, bytes = 0;
   for (int64_t w0 = (int64_t)blockIdx.x * kWalkThreads; w0 < n; w0 += (int64_t)gridDim.x * kWalkThreads) {
     const int64_t w = w0 + threadIdx.x;
@@ -89,12 +100,12 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk(GraphDev G, csaw_graph_vi
         if (act && v >= 0) {
           int64_t o;
           int32_t d;
-          float T = 0.0f;
-          if (KIND == kF32 && vt) {
+          uint32_t T = 0;
+          if (KIND != kUniform && vt) {
             const VertexRecord rec = vt[v];  // offset, degree and CDF total in one 16-byte load
             o = rec.off;
             d = rec.deg;
-            T = __uint_as_float(rec.total);
+            T = rec.total;
           } else {
             o = G.off[v];
             d = (int32_t)(G.off[v + 1] - o);  // same line as off[v]
@@ -103,8 +114,10 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk(GraphDev G, csaw_graph_vi
             v = -1;
           } else {
             const U4 r = philox4x32_10((uint32_t)t, (uint32_t)w, (uint32_t)(w >> 32), c3, k0, k1);
-            const int32_t pick =
-                KIND == kF32 && vt ? walk_pick_f32(G, o, d, T, r) : walk_pick<KIND>(G, hv, bias, v, o, d, r);
+            const int32_t pick = KIND != kUniform && vt
+                                     ? (KIND == kF32 ? walk_pick_f32(G, o, d, __uint_as_float(T), r)
+                                                     : walk_pick_i32(G, ck, o, d, T, r))
+                                     : walk_pick<KIND>(G, hv, bias, v, o, d, r);
             v = G.adj[o + pick];
             ++moved;
             // algorithmic bytes of the step: offset 8 + degree 4 + picked id 4 +
@@ -185,7 +198,7 @@ extern "C" int csaw_walk_plan_create(csaw_graph* g, int32_t bias, uint64_t seed,
     if (seeds[i] < 0 || seeds[i] >= g->V) return CSAW_ERR_INVALID_ARG;
   // walks search the CDF cache (FP32 CDF or integer cache) through its index
   CSAW_TRY(graph_prepare_index(g, bias));
-  if (bias == CSAW_BIAS_WEIGHT) CSAW_TRY(graph_prepare_vertex_records(g, bias));
+  CSAW_TRY(graph_prepare_vertex_records(g, bias));
   csaw_walk_plan* p = new csaw_walk_plan();
   p->g = g;
   p->bias = bias;
