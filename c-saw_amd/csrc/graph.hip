@@ -110,15 +110,44 @@ __global__ void k_weights(uint32_t k0, uint32_t k1, int64_t E, float* __restrict
 }
 
 // Sequential FP32 CDF per vertex, strictly left to right (one thread per
-// vertex; never a parallel scan, whose rounding would differ).
+// vertex; never a parallel scan, whose rounding would differ).  Vertices of
+// degree above skip_above are left to k_fp32_cdf_hubs.
 __global__ void k_fp32_cdf(const int64_t* __restrict__ off, const float* __restrict__ w, int64_t V,
-                           float* __restrict__ cdf) {
+                           int64_t skip_above, float* __restrict__ cdf) {
   for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
     float acc = 0.0f;
     const int64_t b = off[v], e = off[v + 1];
+    if (e - b > skip_above) continue;
     for (int64_t i = b; i < e; ++i) {
       acc = __fadd_rn(acc, w[i]);
       cdf[i] = acc;
+    }
+  }
+}
+
+// The same left-to-right fold for the hubs, one wave per hub: the wave loads
+// 64 weights at a time (the next 64 in flight meanwhile) and every lane
+// applies the 64 additions in order to the same running sum, keeping the one
+// at its own position.
+__global__ __launch_bounds__(256) void k_fp32_cdf_hubs(const int64_t* __restrict__ off, const float* __restrict__ w,
+                                                       const int32_t* __restrict__ hubs, int32_t n_hubs,
+                                                       float* __restrict__ cdf) {
+  const int lane = lane_id();
+  for (int64_t h = (int64_t)blockIdx.x * 4 + threadIdx.x / kWave; h < n_hubs; h += (int64_t)gridDim.x * 4) {
+    const int32_t v = hubs[h];
+    const int64_t b = off[v], e = off[v + 1];
+    float acc = 0.0f;
+    float x = b + lane < e ? w[b + lane] : 0.0f;
+    for (int64_t base = b; base < e; base += kWave) {
+      const float nx = base + kWave + lane < e ? w[base + kWave + lane] : 0.0f;
+      const int cnt = (int)min((int64_t)kWave, e - base);
+      float mine = 0.0f;
+      for (int j = 0; j < cnt; ++j) {
+        acc = __fadd_rn(acc, __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), j)));
+        if (lane == j) mine = acc;
+      }
+      if (lane < cnt) cdf[base + lane] = mine;
+      x = nx;
     }
   }
 }
@@ -256,6 +285,28 @@ static int finish_graph(csaw_graph* g) {
   return CSAW_OK;
 }
 
+// Vertices of degree above kHubMin, sorted by id, with their pool offsets.
+static int ensure_hub_list(csaw_graph* g) {
+  if (g->n_hubs >= 0) return CSAW_OK;
+  std::vector<int32_t> deg(g->V);
+  if (g->V > 0) CSAW_HIP(hipMemcpy(deg.data(), g->deg, sizeof(int32_t) * g->V, hipMemcpyDeviceToHost));
+  std::vector<int32_t> vid;
+  std::vector<int64_t> pool{0};
+  for (int64_t v = 0; v < g->V; ++v)
+    if (deg[v] > kHubMin) {
+      vid.push_back((int32_t)v);
+      pool.push_back(pool.back() + deg[v]);
+    }
+  CSAW_HIP(hipMalloc(&g->hub_vid, sizeof(int32_t) * (vid.size() + 1)));
+  CSAW_HIP(hipMalloc(&g->hub_pool, sizeof(int64_t) * pool.size()));
+  if (!vid.empty())
+    CSAW_HIP(hipMemcpy(g->hub_vid, vid.data(), sizeof(int32_t) * vid.size(), hipMemcpyHostToDevice));
+  CSAW_HIP(hipMemcpy(g->hub_pool, pool.data(), sizeof(int64_t) * pool.size(), hipMemcpyHostToDevice));
+  g->n_hubs = (int32_t)vid.size();
+  g->hub_entries = pool.back();
+  return CSAW_OK;
+}
+
 int graph_prepare_bias(csaw_graph* g, int32_t bias) {
   if (bias == CSAW_BIAS_WEIGHT) {
     if (!g->w) {
@@ -268,8 +319,15 @@ int graph_prepare_bias(csaw_graph* g, int32_t bias) {
       CSAW_HIP(hipMalloc(&g->cdf, sizeof(float) * n));
       CSAW_HIP(hipMemsetAsync(g->cdf, 0, sizeof(float) * n, g->stream));
       if (g->V > 0) {
-        k_fp32_cdf<<<grid_for(g->V, 256, g->num_cu), 256, 0, g->stream>>>(g->off, g->w, g->V, g->cdf);
+        CSAW_TRY(ensure_hub_list(g));
+        k_fp32_cdf<<<grid_for(g->V, 256, g->num_cu), 256, 0, g->stream>>>(g->off, g->w, g->V, (int64_t)kHubMin,
+                                                                           g->cdf);
         CSAW_LAUNCH_CHECK();
+        if (g->n_hubs > 0) {
+          const int grid = (int)std::min<int64_t>((g->n_hubs + 3) / 4, (int64_t)g->num_cu * 8);
+          k_fp32_cdf_hubs<<<grid, 256, 0, g->stream>>>(g->off, g->w, g->hub_vid, g->n_hubs, g->cdf);
+          CSAW_LAUNCH_CHECK();
+        }
       }
       CSAW_HIP(hipStreamSynchronize(g->stream));
     }
@@ -277,24 +335,7 @@ int graph_prepare_bias(csaw_graph* g, int32_t bias) {
   }
   if (bias != CSAW_BIAS_DEGREE && bias != CSAW_BIAS_USER) return CSAW_OK;
   const int kind = bias == CSAW_BIAS_DEGREE ? 0 : 1;
-  if (g->n_hubs < 0) {
-    std::vector<int32_t> deg(g->V);
-    if (g->V > 0) CSAW_HIP(hipMemcpy(deg.data(), g->deg, sizeof(int32_t) * g->V, hipMemcpyDeviceToHost));
-    std::vector<int32_t> vid;
-    std::vector<int64_t> pool{0};
-    for (int64_t v = 0; v < g->V; ++v)
-      if (deg[v] > kHubMin) {
-        vid.push_back((int32_t)v);
-        pool.push_back(pool.back() + deg[v]);
-      }
-    g->n_hubs = (int32_t)vid.size();
-    g->hub_entries = pool.back();
-    CSAW_HIP(hipMalloc(&g->hub_vid, sizeof(int32_t) * (vid.size() + 1)));
-    CSAW_HIP(hipMalloc(&g->hub_pool, sizeof(int64_t) * pool.size()));
-    if (!vid.empty())
-      CSAW_HIP(hipMemcpy(g->hub_vid, vid.data(), sizeof(int32_t) * vid.size(), hipMemcpyHostToDevice));
-    CSAW_HIP(hipMemcpy(g->hub_pool, pool.data(), sizeof(int64_t) * pool.size(), hipMemcpyHostToDevice));
-  }
+  CSAW_TRY(ensure_hub_list(g));
   if (g->hub_cdf[kind] || g->n_hubs == 0) return CSAW_OK;
   std::vector<int64_t> pool(g->n_hubs + 1);
   CSAW_HIP(hipMemcpy(pool.data(), g->hub_pool, sizeof(int64_t) * pool.size(), hipMemcpyDeviceToHost));

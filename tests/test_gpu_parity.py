@@ -268,6 +268,33 @@ def test_walk_matches_oracle(C, O, g14, bias):
     assert np.array_equal(W, Wo) and steps == so
 
 
+@pytest.mark.parametrize("weights", ["random", "one_big"])
+def test_fp32_cdf_of_a_hub(C, O, weights):
+    # a star whose centre has 5,000 neighbours (a hub: its FP32 CDF is folded
+    # by one wave, the leaves' by one thread each); with one big weight
+    # first, 1e8 + 1 rounds back to 1e8 at every step of a left-to-right
+    # fold, so any other summation order changes the total and the draws
+    d = 5000
+    off = np.concatenate([[0], d + np.arange(0, d + 1)]).astype(np.int64)
+    adj = np.concatenate([np.arange(1, d + 1), np.zeros(d)]).astype(np.int32)
+    rng = np.random.default_rng(3)
+    w = rng.uniform(1e-3, 1.0, 2 * d).astype(np.float32)
+    if weights == "one_big":
+        w[:d] = 1.0
+        w[0] = 1e8
+    g = C.Graph.from_csr(off, adj, w)
+    og = O.OracleGraph(off, adj, weights=w)
+    seeds = np.zeros(20000, np.int32)
+    W, steps = C.random_walk(g, seeds, 2, bias=2)
+    Wo, so = O.walk(og, seeds, 2, bias=2)
+    assert np.array_equal(W, Wo) and steps == so
+    if weights == "one_big":
+        assert (W[:, 1] == 1).all()  # C[0] = 1e8 = the total: every draw picks neighbour 0
+    got = C.sample(g, seeds[:300], [20], bias=2)
+    want = O.sample(og, seeds[:300], [20], bias=2)
+    same(got, want, "hub fp32")
+
+
 def test_walk_more_walkers_than_lanes(C, O, g14):
     # 600,000 walkers outnumber the lanes of one launch (256 CUs x 8 blocks
     # x 256), so blocks take a second slice of walkers; a length of 37
