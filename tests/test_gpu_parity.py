@@ -259,7 +259,7 @@ def test_cdf_cache_edge_cases(C, O):
     _check_cache_plans(C, O, g, og, specs, "edge")
 
 
-@pytest.mark.parametrize("bias", [0, 1, 2])
+@pytest.mark.parametrize("bias", [0, 1, 2, 3])
 def test_walk_matches_oracle(C, O, g14, bias):
     g, og, *_ = g14
     seeds = g.seeds(3000, request=5)
