@@ -89,7 +89,9 @@ using namespace csaw;
 extern "C" int csaw_mdrw(csaw_graph* g, uint64_t seed, uint32_t request_id, int32_t samples, int32_t frontier_size,
                          int32_t depth, const int32_t* seeds, int32_t* src, int32_t* dst, int64_t* offsets,
                          csaw_stats* stats) {
+  // request ids share counter word 3 with the domain tag in bits 28-31
   if (!g || samples < 0 || frontier_size < 1 || frontier_size > kMdrwMaxFs || depth < 0 || !offsets ||
+      request_id >= (1u << 28) ||
       (samples > 0 && (!seeds || (depth > 0 && (!src || !dst)))))
     return CSAW_ERR_INVALID_ARG;
   const int64_t ns = (int64_t)samples * frontier_size;

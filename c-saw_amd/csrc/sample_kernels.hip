@@ -283,32 +283,37 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, int64_t n) {
 
 // ----------------------------------------------------------- expand plain
 // gs: group size for the hop's fanout (group_size_for; 0 = the whole-wave
-// path), used for the kinds that need no per-entry CDF build (uniform, FP32).
-template <int KIND>
+// path), used for the kinds that need no per-entry CDF build (uniform, FP32,
+// and integer biases when the CDF cache is on).  IDX: the graph view carries
+// the CDF cache and its 16-ary index (searched through the index).
+template <int KIND, bool IDX>
 __global__ __launch_bounds__(256) void k_expand_plain(GraphDev G, csaw_graph_view hv, const ReqDev* __restrict__ R,
                                                       const int32_t* __restrict__ gs2req, HopDev H, int gs) {
+  constexpr bool kGroupOk = KIND != kInt || IDX;            // no per-entry CDF build needed
+  constexpr bool kGroupLds = KIND == kF32 || (KIND == kInt && IDX);  // small cached CDFs staged in LDS
   __shared__ uint64_t cdf[KIND == kInt ? 4 * kWaveIntCap : 1];
   __shared__ uint32_t bms[4 * (kWaveBitmapCap / 32)];
-  __shared__ uint32_t glds[KIND == kF32 ? 4 * kGroupLdsWords : 1];
+  __shared__ uint32_t glds[kGroupLds ? 4 * kGroupLdsWords : 1];
   const int64_t n = *H.n_ptr;
   const int wid = threadIdx.x / kWave;
   const int64_t nw = (int64_t)gridDim.x * 4;
   // batch size adapts to the frontier: one entry (one group round) per wave
   // when it is small (all waves busy), up to 64 entries per metadata round
   // when it is large
-  const int64_t per = KIND != kInt && gs > 0 ? kWave / gs : 1;
+  const bool grp = kGroupOk && gs > 0;
+  const int64_t per = grp ? kWave / gs : 1;
   const int64_t bs = min((int64_t)kWave, max(per, (n + nw - 1) / nw));
-  uint32_t* gl = KIND == kF32 ? glds + wid * kGroupLdsWords : nullptr;
+  uint32_t* gl = kGroupLds ? glds + wid * kGroupLdsWords : nullptr;
   for (int64_t e0 = (blockIdx.x * 4 + wid) * bs; e0 < n; e0 += nw * bs) {
     const int cnt = (int)min(bs, n - e0);
-    if constexpr (KIND != kInt) {
-      if (gs > 0) {
+    if constexpr (kGroupOk) {
+      if (grp) {
         group_expand_batch<KIND>(G, hv, R, gs2req, H, nullptr, e0, cnt, gs, gl);
         continue;
       }
     }
-    wave_expand_batch<KIND, false>(G, hv, R, gs2req, H, nullptr, e0, cnt, KIND == kInt ? cdf + wid * kWaveIntCap : nullptr,
-                                   bms + wid * (kWaveBitmapCap / 32));
+    wave_expand_batch<KIND, IDX>(G, hv, R, gs2req, H, nullptr, e0, cnt, KIND == kInt ? cdf + wid * kWaveIntCap : nullptr,
+                                 bms + wid * (kWaveBitmapCap / 32));
   }
 }
 
@@ -707,9 +712,13 @@ int launch_expand_plain(const LaunchCfg& L, int kind, const GraphDev& G, const c
   int64_t nwg = (max_entries + 3) / 4;
   int grid = (int)std::max<int64_t>(1, std::min<int64_t>(nwg, grid_of(L, L.expand_per_cu)));
   if (L.grid_override > 0) grid = L.grid_override;
-  if (kind == kUniform) k_expand_plain<kUniform><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, gs);
-  else if (kind == kInt) k_expand_plain<kInt><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, 0);
-  else k_expand_plain<kF32><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, gs);
+  // the CDF cache (and its index) is in the view: plain sampling with cdf_cache
+  const bool idx = G.cdf_idx || G.int_idx32[0] || G.int_idx32[1] || G.int_idx[0] || G.int_idx[1];
+  if (kind == kUniform) k_expand_plain<kUniform, false><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, gs);
+  else if (kind == kInt && idx) k_expand_plain<kInt, true><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, gs);
+  else if (kind == kInt) k_expand_plain<kInt, false><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, 0);
+  else if (idx) k_expand_plain<kF32, true><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, gs);
+  else k_expand_plain<kF32, false><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, gs);
   CSAW_LAUNCH_CHECK();
   return CSAW_OK;
 }

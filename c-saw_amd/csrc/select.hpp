@@ -115,6 +115,11 @@ struct Picker {
                                           (uint32_t)x);
       if (mode == kPickIntIdx)
         return cdf_index_search<uint64_t>(static_cast<const uint64_t*>(C), static_cast<const uint64_t*>(K), E, o, d, x);
+    } else {
+      // without the index, a uint32 cache is still searched with its own key
+      // width (x < T < 2^32)
+      if (mode == kPickInt32Idx)
+        return (int32_t)upper_bound_dev<uint32_t>(static_cast<const uint32_t*>(C) + o, d, (uint32_t)x);
     }
     return (int32_t)upper_bound_dev<uint64_t>(static_cast<const uint64_t*>(C) + o, d, x);
   }

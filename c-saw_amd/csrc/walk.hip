@@ -193,7 +193,9 @@ struct csaw_walk_plan {
 
 extern "C" int csaw_walk_plan_create(csaw_graph* g, int32_t bias, uint64_t seed, uint32_t request_id,
                                      const int32_t* seeds, int64_t n_walkers, int32_t len, csaw_walk_plan** out) {
-  if (!g || !out || n_walkers < 0 || len < 0 || (n_walkers > 0 && !seeds) || bias < 0 || bias > CSAW_BIAS_USER)
+  // request ids share counter word 3 with the domain tag in bits 28-31
+  if (!g || !out || n_walkers < 0 || len < 0 || (n_walkers > 0 && !seeds) || bias < 0 || bias > CSAW_BIAS_USER ||
+      request_id >= (1u << 28))
     return CSAW_ERR_INVALID_ARG;
   *out = nullptr;
   for (int64_t i = 0; i < n_walkers; ++i)

@@ -163,6 +163,13 @@ def test_invalid_arguments(C, g14):
         C.Graph.from_csr(np.array([0, 2, 1], np.int64), np.array([1, 0], np.int32))
     with pytest.raises(C.CsawError):
         C.Graph.from_csr(np.array([0, 1, 2], np.int64), np.array([1, 5], np.int32))
+    # request ids share Philox counter word 3 with the domain tag (bits 28-31)
+    with pytest.raises(C.CsawError):
+        C.random_walk(g, g.seeds(4), 3, bias=0, request=1 << 28)
+    with pytest.raises(C.CsawError):
+        C.mdrw(g, g.seeds(8), 4, 2, 3, request=1 << 28)
+    with pytest.raises(C.CsawError):
+        C.sample(g, g.seeds(4), [2], request=1 << 20)
 
 
 def test_batched_equals_plain_and_oracle(C, O, g14):
@@ -213,16 +220,20 @@ def _cache_specs(seeds_of, fans, biases, dedups):
 
 
 def _check_cache_plans(C, O, g, og, specs, what, cdf_cache=True):
-    """Every forced variant, the heuristic and random schedules, with the
-    CDF cache (index searches, sub-wave groups) against the oracle."""
+    """Every forced variant, the heuristic and random schedules, and the
+    plain (one request at a time) sampler, with the CDF cache (index
+    searches, sub-wave groups) against the oracle."""
     want = [O.sample(og, s["seeds"], s["fanouts"], bias=s["bias"] if s["bias"] != 3 else O.BIAS_USER,
                      dedup=s["dedup"], request=s["request"]) for s in specs]
-    runs = [(C.SCHED_FORCED, v) for v in range(C.NUM_VARIANTS)] + [(C.SCHED_HEURISTIC, 0), (C.SCHED_RANDOM, 0)]
-    for sched, v in runs:
-        p = C.Plan(g, specs, mode=C.MODE_BATCHED, sched=sched, variant=v, cdf_cache=cdf_cache, explore_key=5)
+    runs = [(C.MODE_BATCHED, C.SCHED_FORCED, v) for v in range(C.NUM_VARIANTS)]
+    runs += [(C.MODE_BATCHED, C.SCHED_HEURISTIC, 0), (C.MODE_BATCHED, C.SCHED_RANDOM, 0),
+             (C.MODE_PLAIN, C.SCHED_HEURISTIC, 0)]
+    for mode, sched, v in runs:
+        p = C.Plan(g, specs, mode=mode, sched=sched, variant=v, cdf_cache=cdf_cache, explore_key=5)
         p.run()
         for i, s in enumerate(specs):
-            same(p.result(i), want[i], f"{what} sched {sched} variant {v} req {i} {s['fanouts']} b{s['bias']}")
+            same(p.result(i), want[i],
+                 f"{what} mode {mode} sched {sched} variant {v} req {i} {s['fanouts']} b{s['bias']}")
         p.close()
 
 
@@ -257,6 +268,35 @@ def test_cdf_cache_edge_cases(C, O):
     seeds = np.array([0, 1, 2, 3, 4, 5, 6, 0, 6, 2], np.int32)
     specs = _cache_specs(lambda i: seeds, [[1], [2], [3], [2, 2, 2], [64, 1]], [0, 1, 2], [0, 1])
     _check_cache_plans(C, O, g, og, specs, "edge")
+
+
+@pytest.mark.parametrize("w,F,want", [([0, 1, 0, 0], 3, [1, 2, 3]), ([0, 0, 0, 1], 3, [3, 0, 1]),
+                                      ([0, 0, 0, 0, 0, 1, 0], 4, [5, 6, 0, 1])])
+def test_fallback_known_answers(C, O, w, F, want):
+    # one positive weight at index p: slot 0 takes p, every later slot misses
+    # its 400 attempts and takes the first untaken index cyclically above p
+    d = len(w)
+    p = w.index(1)
+    lists = [list(range(1, d + 1))] + [[] for _ in range(d)]
+    lists[1 + p] = list(range(d + 1, d + 6))  # degree bias: the only neighbour with out-edges
+    lists += [[] for _ in range(5)]
+    off = np.concatenate([[0], np.cumsum([len(l) for l in lists])]).astype(np.int64)
+    adj = np.array([x for l in lists for x in l], np.int32)
+    wt = np.zeros(len(adj), np.float32)
+    wt[:d] = w
+    wt[d:] = 1.0
+    g = C.Graph.from_csr(off, adj, wt)
+    seeds = np.zeros(100, np.int32)
+    exp = np.tile(np.array(want, np.int32) + 1, 100)
+    for bias in (1, 2):
+        assert np.array_equal(C.sample(g, seeds, [F], bias=bias)["dst"], exp), ("plain", bias)
+        for v in range(C.NUM_VARIANTS):
+            assert np.array_equal(C.sample(g, seeds, [F], bias=bias, sched=C.SCHED_FORCED, variant=v)["dst"], exp)
+        for mode in (C.MODE_PLAIN, C.MODE_BATCHED):
+            pl = C.Plan(g, [{"seeds": seeds, "fanouts": [F], "bias": bias, "request": 0}], mode=mode, cdf_cache=True)
+            pl.run()
+            assert np.array_equal(pl.result(0)["dst"], exp), ("cache", mode, bias)
+            pl.close()
 
 
 @pytest.mark.parametrize("bias", [0, 1, 2, 3])

@@ -8,7 +8,10 @@ Pins, by part of the oracle:
   selection law      brute-force exact successive-sampling probabilities on tiny graphs, chi-square
   multi-hop          structural invariants; sequential == OpenMP restatement
   RMAT               symmetric, sorted, unique, loop-free; committed digest of the scale-16 CSR
-  walk / MDRW        continuity invariants and step-frequency chi-square
+  walk               continuity invariants and step-frequency chi-square
+  MDRW               exact law of 2-step edge sequences by enumeration, chi-square (and the
+                     same data rejecting a uniform-slot law); all-zero-bias frontiers
+  reselect fallback  closed-form answers with a single positive weight (FP32 and degree bias)
 """
 import hashlib
 import json
@@ -212,6 +215,48 @@ def test_with_replacement_law(O):
     assert stats.chisquare(obs, exp).pvalue > 1e-4
 
 
+def directed_star(d, extra=None):
+    """vertex 0 -> 1..d; extra: {neighbour index: out-degree} gives neighbour
+    1+i that many leaves of its own (the leaves come after 1..d)."""
+    lists = [list(range(1, d + 1))] + [[] for _ in range(d)]
+    nxt = d + 1
+    for i, k in (extra or {}).items():
+        lists[1 + i] = list(range(nxt, nxt + k))
+        lists.extend([] for _ in range(k))
+        nxt += k
+    off = np.concatenate([[0], np.cumsum([len(l) for l in lists])]).astype(np.int64)
+    adj = np.array([x for l in lists for x in l], np.int32)
+    return off, adj
+
+
+# Closed-form answers of the reselect fallback.  With a single positive
+# weight at index p, every attempt of every slot draws p: slot 0 takes p,
+# each later slot misses all 400 attempts and takes the first untaken index
+# scanning upward cyclically from p.
+@pytest.mark.parametrize("w,F,want", [
+    ([0, 1, 0, 0], 3, [1, 2, 3]),
+    ([0, 0, 1, 0, 0], 3, [2, 3, 4]),
+    ([0, 0, 0, 1], 3, [3, 0, 1]),
+    ([0, 0, 0, 0, 0, 1, 0], 4, [5, 6, 0, 1]),
+])
+def test_fallback_known_answers(O, w, F, want):
+    d = len(w)
+    off, adj = directed_star(d)
+    # FP32 weights: only vertex 0's entries carry weights
+    g = O.OracleGraph(off, adj, weights=np.array(w, np.float32))
+    for s in range(20):
+        assert list(O.select_source(g, 0, F, bias=O.BIAS_WEIGHT, sample=s, pos=s)) == want
+    r = O.sample(g, np.zeros(30, np.int32), [F], bias=O.BIAS_WEIGHT)
+    assert (r["dst"].reshape(30, F) == np.array(want) + 1).all()
+    # integer (degree) bias: the neighbour at index p is the only one with
+    # out-neighbours, so its degree is the only positive bias
+    p = w.index(1)
+    off, adj = directed_star(d, {p: 5})
+    g = O.OracleGraph(off, adj)
+    r = O.sample(g, np.zeros(30, np.int32), [F], bias=O.BIAS_DEGREE)
+    assert (r["dst"].reshape(30, F) == np.array(want) + 1).all()
+
+
 def test_fallback_path_distinct(O):
     # hub neighbour with 99.9% of the weight: later slots exhaust the 400
     # attempts and take the cyclic fallback; picks stay distinct, k = F
@@ -313,6 +358,84 @@ def test_walk_step_law(O):
     W, _ = O.walk(g, np.zeros(N, np.int32), 1, bias=O.BIAS_WEIGHT)
     obs = np.bincount(W[:, 1] - 1, minlength=4).astype(float)
     assert stats.chisquare(obs, w_nb / w_nb.sum() * N).pvalue > 1e-4
+
+
+# ----------------------------------------------------------- MDRW law (exact)
+MDRW_LISTS = [[1, 2], [0, 2, 3, 4], [3], [], [0, 1, 2]]  # degrees 2, 4, 1, 0, 3
+
+
+def mdrw_law(lists, front, depth, slot_law="bias"):
+    """Exact probability of every emitted edge sequence of a multi-dimensional
+    walk, by enumeration: per step slot q with probability VertexBias(front[q])
+    / sum (VertexBias = out-degree; "uniform" for the wrong law 1/FS), then a
+    uniform neighbour nb of u = front[q]; emit (u, nb); front[q] = nb."""
+    if depth == 0:
+        return {(): 1.0}
+    degs = [len(lists[v]) for v in front]
+    tot = sum(degs)
+    out = {}
+    if tot == 0:  # uniform slot, degree 0: the step is consumed, nothing emitted
+        for seq, p in mdrw_law(lists, front, depth - 1, slot_law).items():
+            out[seq] = out.get(seq, 0.0) + p
+        return out
+    for q, v in enumerate(front):
+        pq = degs[q] / tot if slot_law == "bias" else 1.0 / len(front)
+        if degs[q] == 0:
+            continue
+        for nb in lists[v]:
+            nf = list(front)
+            nf[q] = nb
+            for seq, p in mdrw_law(lists, nf, depth - 1, slot_law).items():
+                k = ((v, nb),) + seq
+                out[k] = out.get(k, 0.0) + pq / degs[q] * p
+    return out
+
+
+def _chi2_cells(law, counts, N):
+    keys = sorted(law)
+    exp = np.array([law[k] * N for k in keys])
+    obs = np.array([counts.get(k, 0) for k in keys], float)
+    assert sum(counts.values()) == N and set(counts) <= set(law)
+    small = exp < 5  # pool sparse cells
+    if small.any():
+        exp = np.append(exp[~small], exp[small].sum())
+        obs = np.append(obs[~small], obs[small].sum())
+    return stats.chisquare(obs, exp * obs.sum() / exp.sum()).pvalue
+
+
+def test_mdrw_law_exact(O):
+    # slot chosen proportionally to VertexBias (ITS_MDRW, functions.cuh:1014-1060),
+    # neighbour uniform (rand_integer, functions.cuh:210-216), slot replaced
+    lists = MDRW_LISTS
+    off = np.concatenate([[0], np.cumsum([len(l) for l in lists])]).astype(np.int64)
+    adj = np.array([x for l in lists for x in l], np.int32)
+    g = O.OracleGraph(off, adj)
+    front, D, N = [0, 1, 4], 2, 60000
+    src, dst, offs = O.mdrw(g, np.tile(np.array(front, np.int32), N), N, len(front), D)
+    counts = {}
+    for s in range(N):
+        e = range(offs[s], offs[s + 1])
+        k = tuple((int(src[i]), int(dst[i])) for i in e)
+        counts[k] = counts.get(k, 0) + 1
+    assert _chi2_cells(mdrw_law(lists, front, D), counts, N) > 1e-4
+    # the same data rejects a uniform slot law (the test has the power to
+    # catch that mistake)
+    wrong = mdrw_law(lists, front, D, slot_law="uniform")
+    assert _chi2_cells(wrong, {k: c for k, c in counts.items() if k in wrong}, sum(
+        c for k, c in counts.items() if k in wrong)) < 1e-12
+
+
+def test_mdrw_all_zero_bias_consumes_steps(O):
+    # a frontier of degree-0 vertices: uniform slot, no neighbour, no edge
+    lists = MDRW_LISTS
+    off = np.concatenate([[0], np.cumsum([len(l) for l in lists])]).astype(np.int64)
+    adj = np.array([x for l in lists for x in l], np.int32)
+    g = O.OracleGraph(off, adj)
+    src, dst, offs = O.mdrw(g, np.full(3 * 10, 3, np.int32), 10, 3, 6)
+    assert len(src) == 0 and (offs == 0).all()
+    # one positive-bias slot among zeros: every step takes it
+    src, dst, offs = O.mdrw(g, np.tile(np.array([3, 2, 3], np.int32), 10), 10, 3, 1)
+    assert list(src) == [2] * 10 and list(dst) == [3] * 10
 
 
 def test_mdrw_invariants(O):
