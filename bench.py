@@ -300,6 +300,42 @@ def bench_stream(a, C, O, CM):
          schedules_bit_equal=bool(equal), cost_model=meta)
 
 
+def probe_lib():
+    import ctypes
+    L = ctypes.CDLL(os.path.join(ROOT, "c-saw_amd", "lib", "libcsaw_gather.so"))
+    L.csaw_gather_probe.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
+    L.csaw_copy_probe.argtypes = [ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+    return L
+
+
+def gather_ceiling(array_mib=512, reads=64 << 20, wg_per_cu=8, iters=5):
+    """Random-read ceiling: independent random 4-byte reads per second from an
+    array of array_mib MiB (512 MiB: the RMAT-22 neighbour array), measured by
+    the library's own probe kernel (csrc/gather_probe.hip)."""
+    import ctypes
+    L = probe_lib()
+    ms, n = ctypes.c_double(), ctypes.c_int64()
+    rc = L.csaw_gather_probe(array_mib << 20, reads, wg_per_cu, iters, ctypes.byref(ms), ctypes.byref(n))
+    if rc != 0:
+        raise RuntimeError(f"csaw_gather_probe: hip error {rc}")
+    return {"array_mib": array_mib, "random_reads": n.value, "ms": ms.value, "wg_per_cu": wg_per_cu,
+            "reads_per_s": n.value / (ms.value * 1e-3), "line_bytes_per_s_at_64B": n.value * 64 / (ms.value * 1e-3)}
+
+
+def probe(a):
+    import ctypes
+    if a.only == "copy":
+        # 256 MiB read + 256 MiB written per launch: scales FETCH_SIZE / WRITE_SIZE
+        ms = ctypes.c_double()
+        rc = probe_lib().csaw_copy_probe(256 << 20, a.steps, ctypes.byref(ms))
+        if rc != 0:
+            raise RuntimeError(f"csaw_copy_probe: hip error {rc}")
+        return {"copy_bytes_read": 256 << 20, "copy_bytes_written": 256 << 20, "launches": a.steps, "ms": ms.value,
+                "gbs": 2 * (256 << 20) / (ms.value * 1e-3) / 1e9}
+    return gather_ceiling(min(a.gather_mib, 8191), iters=a.steps)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -323,34 +359,8 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported "
                                    "(the path would shard by seeds with no exchange between GPUs)"}))
         sys.exit(2)
-    if a.only == "copy":
-        import torch
-        x = torch.ones(64 << 20, dtype=torch.int32, device="cuda")
-        y = torch.empty_like(x)
-        for _ in range(a.steps):
-            torch.add(x, 1, out=y)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4,
-                          "launches": a.steps}))
-        return
-    if a.only == "gather":
-        # random-read ceiling: 64M independent random 4-byte reads from a
-        # 512 MiB array (the size of the RMAT-22 neighbour array) by default
-        import torch
-        x = torch.ones(min(a.gather_mib, 8191) << 18, dtype=torch.int32, device="cuda")
-        idx = torch.randint(0, x.numel(), (64 << 20,), device="cuda", dtype=torch.int32)
-        y = x[idx]
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(a.steps):
-            y = x[idx]
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / a.steps
-        print(json.dumps({"array_mib": x.numel() >> 18, "random_reads": idx.numel(), "ms": ms,
-                          "reads_per_s": idx.numel() / (ms * 1e-3),
-                          "line_bytes_per_s_at_64B": idx.numel() * 64 / (ms * 1e-3)}))
+    if a.only in ("copy", "gather"):
+        print(json.dumps(probe(a)))
         return
     import __graft_entry__ as ge
     C = ge.load_csaw()
@@ -392,24 +402,55 @@ def main():
         print(json.dumps({"only": a.only, "ms_per_step": t * 1e3, "edges": e, "value": e / t}))
         return
 
+    # one-time preprocessing, timed apart from the steps: the per-bias data
+    # both samplers need (FP32 CDF of the weights, hub CDFs), then the CDF
+    # cache of the batched sampler (integer CDF cache = the reference's Cp,
+    # 16-ary search indexes, vertex records)
+    tiny = [{"seeds": specs[0]["seeds"][:1], "fanouts": [1], "bias": b, "request": 0} for b in (1, 2)]
+    t0 = time.perf_counter()
+    C.Plan(g, tiny, mode=C.MODE_PLAIN).close()
+    prep_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    C.Plan(g, tiny, mode=C.MODE_PLAIN, cdf_cache=True).close()
+    cache_ms = (time.perf_counter() - t0) * 1e3
+    log(f"one-time: bias data {prep_ms:.1f} ms, CDF cache {cache_ms:.1f} ms")
+
+    # the same 256 requests through four executors: what each of the cache,
+    # batching and the learned schedule adds
     plain = C.Plan(g, specs, mode=C.MODE_PLAIN)
-    batched = C.Plan(g, specs, mode=C.MODE_BATCHED, sched=C.SCHED_MLP, mlp=mlp, use_graph=True, tuning=tuning,
-                     cdf_cache=True, serial_expand=a.serial_expand)
     t_plain = run_timed(C, plain, max(1, min(a.steps, 3)), 1)
-    t_bat = run_timed(C, batched, a.steps, a.warmup)
-    edges = sum(batched.n_edges(i) for i in range(len(specs)))
-    assert edges == sum(plain.n_edges(i) for i in range(len(specs)))
-    same = sum(batched.digest(i) == plain.digest(i) for i in range(len(specs)))
+    want = [plain.digest(i) for i in range(len(specs))]
+    edges = sum(plain.n_edges(i) for i in range(len(specs)))
+    runs = {}
+
+    def measure(name, **kw):
+        p = C.Plan(g, specs, **kw)
+        t = run_timed(C, p, a.steps, a.warmup)
+        eq = sum(p.digest(i) == want[i] for i in range(len(specs)))
+        runs[name] = {"ms_per_step": t * 1e3, "value": edges / t, "requests_equal_to_plain": int(eq)}
+        log(f"{name}: {t * 1e3:.2f} ms, digests equal {eq}/{len(specs)}")
+        return p, t, eq
+
+    p, _, _ = measure("plain_cdf_cache", mode=C.MODE_PLAIN, cdf_cache=True)
+    p.close()
+    p, _, _ = measure("batched_heuristic", mode=C.MODE_BATCHED, sched=C.SCHED_HEURISTIC, use_graph=True,
+                      tuning=tuning, cdf_cache=True, serial_expand=a.serial_expand)
+    p.close()
+    batched, t_bat, same = measure("batched_learned", mode=C.MODE_BATCHED, sched=C.SCHED_MLP, mlp=mlp,
+                                   use_graph=True, tuning=tuning, cdf_cache=True, serial_expand=a.serial_expand)
     log(f"plain {t_plain * 1e3:.1f} ms  batched {t_bat * 1e3:.1f} ms  edges {edges}  digests equal {same}/256")
 
-    # roofline of the dominant kernel kind (event-timed run, same plan settings)
+    # roofline of the dominant kernel kind: event-timed run of the same plan
+    # settings, each expansion kernel timed on the stream it runs on (the
+    # hop's variant kernels side by side, as in the timed steps)
     prof = C.Plan(g, specs, mode=C.MODE_BATCHED, sched=C.SCHED_MLP, mlp=mlp, timing=True, tuning=tuning,
-                  cdf_cache=True)
+                  cdf_cache=True, serial_expand=a.serial_expand)
     prof.run()
     prof.reset_times()
     for _ in range(2):
         prof.run()
     kt = prof.kernel_times(with_bytes=True)
+    prof.close()
     dom = max(kt, key=lambda k: kt[k][0] * kt[k][1])
     avg_ms, nl, avg_b = kt[dom]
     achieved = avg_b / (avg_ms * 1e-3) / 1e9
@@ -419,15 +460,29 @@ def main():
         traffic = json.load(open(tpath)).get(dom)
     log("kernel kinds:", {k: (round(v[0], 3), v[1], round(v[2] / 1e6, 1)) for k, v in kt.items()})
 
+    # the plain-uniform roofline instance on the same graph: 1,048,576 seeds,
+    # fanouts {25,10}, uniform bias, plain sampler
+    uspec = [{"seeds": g.seeds(1 << 20), "fanouts": [25, 10], "bias": C.BIAS_UNIFORM, "request": 0}]
+    up = C.Plan(g, uspec, mode=C.MODE_PLAIN)
+    t_u = run_timed(C, up, a.steps, a.warmup)
+    e_u = up.n_edges(0)
+    up.close()
+    uroof, ukt = roofline_of(C.Plan(g, uspec, mode=C.MODE_PLAIN, timing=True))
+    ceiling = gather_ceiling()
+    log(f"uniform instance {t_u * 1e3:.3f} ms ({e_u} edges), {uroof['kernel']} {uroof['frac']:.3f} of HBM peak; "
+        f"random-read ceiling {ceiling['reads_per_s']:.3g}/s")
+
     cpu = None
     if not a.no_cpu:
         O = ge.load_oracle()
-        nthreads = min(16, os.cpu_count() or 1)
+        host = host_cpus()
+        nthreads = host["threads_used"]
         ce, cs, mism, checked = cpu_baseline(C, O, g, specs, batched, a.cpu_sample, nthreads)
         cpu = {"value": ce / cs, "unit": "edges/s", "cores": nthreads, "kind": "port",
                "sample": f"first {a.cpu_sample:.0%} of the seeds of each of the 256 requests "
                          f"({ce} edges, {cs:.1f} s); GPU edges of those samples equal the oracle's in "
-                         f"{checked - mism}/{checked} request-hops"}
+                         f"{checked - mism}/{checked} request-hops",
+               "host": host}
         log("cpu", cpu)
 
     out = {
@@ -441,14 +496,65 @@ def main():
                    "edges_per_step": int(edges)},
         "plain_value": edges / t_plain, "plain_ms_per_step": t_plain * 1e3,
         "speedup_vs_plain": t_plain / t_bat,
+        # the same requests through each executor; every one bit-equal to plain
+        "executors": dict({"plain": {"ms_per_step": t_plain * 1e3, "value": edges / t_plain}}, **runs),
+        "speedup_parts": {"cdf_cache (plain -> plain+cache)": t_plain / (runs["plain_cdf_cache"]["ms_per_step"] * 1e-3),
+                          "batching (plain+cache -> batched+heuristic)":
+                              runs["plain_cdf_cache"]["ms_per_step"] / runs["batched_heuristic"]["ms_per_step"],
+                          "learned schedule (batched+heuristic -> batched+learned)":
+                              runs["batched_heuristic"]["ms_per_step"] / runs["batched_learned"]["ms_per_step"]},
         "bit_exact": {"requests_equal": int(same), "requests": len(specs)},
+        "one_time_ms": {"bias_data (FP32 CDF, hub CDFs)": prep_ms,
+                        "cdf_cache (integer CDF cache, search indexes, vertex records)": cache_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
-                     "algo_bytes_per_launch": avg_b, "avg_launch_ms": avg_ms},
+                     "algo_bytes_per_launch": avg_b, "avg_launch_ms": avg_ms,
+                     "timing": "hipEvents on each expansion kernel's own stream, concurrent schedule"},
+        "uniform_instance": {"workload": "RMAT-22 ef16, 1,048,576 seeds, {25,10}, uniform, plain sampler",
+                             "ms_per_step": t_u * 1e3, "value": e_u / t_u, "edges": int(e_u), "roofline": uroof},
+        "random_read_ceiling": ceiling,
         "cpu_baseline": cpu,
         "cost_model": meta,
     }
     print(json.dumps(out))
+
+
+def host_cpus():
+    """Host cores: nproc, the CPUs this process may run on, the cgroup's CPU
+    quota, and the model name; the baseline uses every CPU the job is
+    granted (affinity and quota)."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except OSError:
+        info["affinity"] = os.cpu_count()
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = int(q) / int(per)
+    except (OSError, ValueError):
+        pass
+    info["cgroup_cpu_quota"] = quota
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                info["model"] = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    # the job's CPU share: affinity, the cgroup quota, and OMP_NUM_THREADS
+    # where the machine sets it to the share it grants (16 per GPU on the
+    # GPU boxes here)
+    n = info["affinity"] or 1
+    if quota:
+        n = min(n, max(1, int(quota)))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    info["omp_num_threads"] = omp
+    if omp and omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    info["threads_used"] = n
+    return info
 
 
 if __name__ == "__main__":

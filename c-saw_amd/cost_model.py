@@ -129,8 +129,18 @@ def regret(data, mlp):
     best = cost[ok].min(1).sum()
     h = cost[ok, heur[ok]].sum()
     m = cost[ok, mc[ok]].sum()
+    # one variant forced for every chunk (the heuristic's choice where that
+    # variant is not valid, as the forced schedule runs it): the best such
+    # fixed policy is the other baseline the learned one must beat
+    single = []
+    for v in range(M.shape[1]):
+        cv = np.where(M[ok, v] > 0, cost[ok, v], cost[ok, heur[ok]])
+        single.append(float(cv.sum()))
+    bv = int(np.argmin(single))
     return {"chunks": int(ok.sum()), "best": float(best), "heuristic": float(h), "mlp": float(m),
             "heuristic_regret": float(h / best - 1.0), "mlp_regret": float(m / best - 1.0),
+            "single_variant": single, "best_single_variant": bv,
+            "best_single_regret": float(single[bv] / best - 1.0),
             "mlp_choice_mix": np.bincount(mc[ok], minlength=M.shape[1]).tolist(),
             "heuristic_choice_mix": np.bincount(heur[ok], minlength=M.shape[1]).tolist()}
 

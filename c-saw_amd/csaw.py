@@ -51,7 +51,7 @@ class Request(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("count_per_cu", i32), ("expand_per_cu", i32), ("thread_per_cu", i32), ("block_per_cu", i32),
-                ("grid_override", i32)]
+                ("grid_override", i32), ("chunk_min", i32), ("group_min", i32)]
 
 
 class PlanOpts(ctypes.Structure):
@@ -371,9 +371,11 @@ class Plan:
             pass
 
 
-def autotune(g, specs, mode=MODE_BATCHED, sched=SCHED_HEURISTIC, use_graph=True):
+def autotune(g, specs, mode=MODE_BATCHED, sched=SCHED_HEURISTIC, use_graph=True, cdf_cache=False, mlp=None):
+    """Launch and tile shapes for these requests: (Tuning, mean ms per run)."""
     reqs, keep = make_requests(specs)
-    opts = PlanOpts(mode=mode, sched=sched, use_graph=1 if use_graph else 0)
+    opts = PlanOpts(mode=mode, sched=sched, use_graph=1 if use_graph else 0, cdf_cache=1 if cdf_cache else 0,
+                    mlp=mlp.h if mlp is not None else None)
     t = Tuning()
     ms = ctypes.c_double()
     _check(_L.csaw_autotune(g.h, reqs, len(specs), ctypes.byref(opts), ctypes.byref(t), ctypes.byref(ms)),

@@ -131,26 +131,31 @@ struct HopDev {
   uint32_t* flags;      // bit 0: chunk overflow
   unsigned long long* bytes;  // algorithmic bytes per kernel kind (null: not counted)
   int32_t expand_waves; // waves of one expand launch (sizes chunks; 0: whole buckets)
+  int32_t chunk_min;    // smallest chunk piece (a power of two, kChunkMin .. kBlockEntries)
+  int32_t group_min;    // smallest sub-wave group (kGroupMin .. 32)
 };
 
 // Entries per chunk for a hop of n frontier entries: whole 1024-entry buckets
 // when the hop gives every expand wave several of them, smaller pieces (down
 // to kChunkMin) when it is small, so that the work still spreads over the
 // waves.
-__host__ __device__ __forceinline__ uint32_t chunk_max_for(int64_t n, int32_t waves) {
+__host__ __device__ __forceinline__ uint32_t chunk_max_for(int64_t n, int32_t waves, int32_t cmin = kChunkMin) {
   if (waves <= 0) return (uint32_t)kBlockEntries;
   const int64_t per = (n + waves - 1) / waves;
-  uint32_t c = (uint32_t)kChunkMin;
+  uint32_t c = (uint32_t)(cmin < kChunkMin ? kChunkMin : cmin);
   while (c < (uint32_t)kBlockEntries && (int64_t)c < per) c <<= 1;
   return c;
 }
 
 // Sub-wave group size for a largest fanout: the fanout itself (at least
-// kGroupMin), so that e.g. fanout 5 runs 12 entries per wave round instead of
-// 8; above 32 one group is the whole wave.
+// gmin, kGroupMin by default), so that e.g. fanout 5 runs 12 entries per
+// wave round instead of 8; above 32 one group is the whole wave.  Any size
+// from kGroupMin to 32 gives the same edges (the autotuner picks gmin).
 constexpr int kGroupMin = 4;
-__host__ __device__ __forceinline__ int group_size_for(int max_f) {
-  return max_f <= kGroupMin ? kGroupMin : (max_f <= 32 ? max_f : kWave);
+__host__ __device__ __forceinline__ int group_size_for(int max_f, int gmin = kGroupMin) {
+  if (gmin < kGroupMin) gmin = kGroupMin;
+  if (gmin > 32) gmin = 32;
+  return max_f <= gmin ? gmin : (max_f <= 32 ? max_f : kWave);
 }
 
 __host__ __device__ __forceinline__ int ceil_log2_u32(uint32_t d) {
@@ -231,24 +236,42 @@ __device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int l) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-// Inclusive wave64 scan.
-__device__ __forceinline__ uint64_t wave_incl_scan_u64(uint64_t x) {
-  const int lane = lane_id();
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    uint64_t y = __shfl_up(x, off, 64);
-    if (lane >= off) x += y;
-  }
+// Inclusive wave64 scans on DPP: four row_shr steps (1, 2, 4, 8) give every
+// lane the prefix within its row of 16 lanes; row_bcast:15 then adds row
+// r's last lane to row r+1 (rows 1 and 3), and row_bcast:31 adds lane 31 to
+// rows 2 and 3.  Lanes whose DPP source lies outside the row (or whose row
+// the row mask disables) receive the `old` operand, 0.  Every lane of the
+// wave must be active (callers run the scan in wave-uniform control flow).
+namespace dpp {
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
+constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143;
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t mov(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint64_t mov64(uint64_t x) {
+  return (uint64_t)mov<CTRL, ROWS>((uint32_t)x) | ((uint64_t)mov<CTRL, ROWS>((uint32_t)(x >> 32)) << 32);
+}
+}  // namespace dpp
+
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t x) {
+  x += dpp::mov<dpp::kRowShr1, 0xf>(x);
+  x += dpp::mov<dpp::kRowShr2, 0xf>(x);
+  x += dpp::mov<dpp::kRowShr4, 0xf>(x);
+  x += dpp::mov<dpp::kRowShr8, 0xf>(x);
+  x += dpp::mov<dpp::kRowBcast15, 0xa>(x);
+  x += dpp::mov<dpp::kRowBcast31, 0xc>(x);
   return x;
 }
 
-__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t x) {
-  const int lane = lane_id();
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    uint32_t y = __shfl_up(x, off, 64);
-    if (lane >= off) x += y;
-  }
+__device__ __forceinline__ uint64_t wave_incl_scan_u64(uint64_t x) {
+  x += dpp::mov64<dpp::kRowShr1, 0xf>(x);
+  x += dpp::mov64<dpp::kRowShr2, 0xf>(x);
+  x += dpp::mov64<dpp::kRowShr4, 0xf>(x);
+  x += dpp::mov64<dpp::kRowShr8, 0xf>(x);
+  x += dpp::mov64<dpp::kRowBcast15, 0xa>(x);
+  x += dpp::mov64<dpp::kRowBcast31, 0xc>(x);
   return x;
 }
 

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -252,6 +253,8 @@ int graph_alloc_common(csaw_graph* g) {
   CSAW_TRY(device_init(&g->num_cu));
   CSAW_HIP(hipGetDevice(&g->device));
   CSAW_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+  const char* keys = getenv("CSAW_INT_CDF_KEYS");
+  g->wide_int_cdf = keys && strcmp(keys, "64") == 0;
   return CSAW_OK;
 }
 
@@ -485,7 +488,7 @@ int graph_prepare_int_cache(csaw_graph* g, int32_t bias) {
   CSAW_HIP(hipMemcpyAsync(&mx, d_max, sizeof(mx), hipMemcpyDeviceToHost, g->stream));
   CSAW_HIP(hipStreamSynchronize(g->stream));
   CSAW_HIP(hipFree(d_max));
-  if (mx < (1ull << 32)) {
+  if (mx < (1ull << 32) && !g->wide_int_cdf) {
     CSAW_HIP(hipMalloc(&g->int_cdf32[kind], sizeof(uint32_t) * n));
     k_narrow_u32<<<grid_for(n, 256, g->num_cu), 256, 0, g->stream>>>(g->int_cdf[kind], n, g->int_cdf32[kind]);
     CSAW_LAUNCH_CHECK();

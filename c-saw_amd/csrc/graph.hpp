@@ -31,6 +31,10 @@ struct csaw_graph {
   hipStream_t stream = nullptr;
   int device = 0;
   int num_cu = 0;
+  // keep integer CDF caches as uint64 even when every total fits 32 bits
+  // (environment CSAW_INT_CDF_KEYS=64 at graph creation; covers the 64-bit
+  // search paths on graphs whose totals are small)
+  bool wide_int_cdf = false;
 
   csaw::GraphDev view(bool use_cache = false) const;
   csaw_graph_view hook_view() const;

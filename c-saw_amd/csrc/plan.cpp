@@ -233,6 +233,8 @@ static HopDev make_hop(csaw_plan* p, Unit& u, int h) {
   H.flags = p->counters + kCtrFlags;
   H.bytes = p->opts.timing ? p->bytes_acc : nullptr;
   H.expand_waves = p->L.num_cu * p->L.expand_per_cu * 4;  // 4 waves per 256-lane workgroup
+  H.chunk_min = p->L.chunk_min;
+  H.group_min = p->L.group_min;
   return H;
 }
 
@@ -278,17 +280,19 @@ static int run_unit(csaw_plan* p, Unit& u, hipStream_t st, TimingHook* th) {
       E.qhead = p->counters + kCtrQhead;
       E.log = p->opts.log_chunks ? p->log : nullptr;
       E.int_cache = p->opts.cdf_cache != 0;
-      if (!th && p->side[0]) {
+      if (p->side[0]) {
         // the hop's expansion kernels (one per variant queue) run side by
         // side on four streams, so one kernel's last chunks overlap the
-        // others' work; all join before the next hop
+        // others' work; all join before the next hop (with timing on, each
+        // kernel's events are recorded on the stream it runs on, so the
+        // times are those of this concurrent schedule)
         hipStream_t ss[1 + csaw_plan::kSide] = {st};
         CSAW_HIP(hipEventRecord(p->fork, st));
         for (int i = 0; i < csaw_plan::kSide; ++i) {
           CSAW_HIP(hipStreamWaitEvent(p->side[i], p->fork, 0));
           ss[1 + i] = p->side[i];
         }
-        CSAW_TRY(launch_expand_chunks(L, G, hv, u.d_req, u.d_gs2req, H, E, u.kinds, nullptr, ss, 1 + csaw_plan::kSide));
+        CSAW_TRY(launch_expand_chunks(L, G, hv, u.d_req, u.d_gs2req, H, E, u.kinds, th, ss, 1 + csaw_plan::kSide));
         for (int i = 0; i < csaw_plan::kSide; ++i) {
           CSAW_HIP(hipEventRecord(p->join[i], p->side[i]));
           CSAW_HIP(hipStreamWaitEvent(st, p->join[i], 0));
@@ -309,7 +313,7 @@ static int run_unit(csaw_plan* p, Unit& u, hipStream_t st, TimingHook* th) {
       int fmax = 0;
       for (int ri : u.reqs)
         if (p->reqs[ri].depth > h) fmax = std::max(fmax, p->reqs[ri].fanouts[h]);
-      const int gs = group_size_for(fmax);
+      const int gs = group_size_for(fmax, L.group_min);
       if (th) th->begin(kKernPlain, st);
       CSAW_TRY(launch_expand_plain(L, kind, G, hv, u.d_req, u.d_gs2req, H, u.nmax[h], gs));
       if (th) th->end(kKernPlain, st);
@@ -343,6 +347,7 @@ extern "C" int csaw_plan_create(csaw_graph* g, const csaw_request* reqs, int32_t
   if (o.mode != CSAW_MODE_PLAIN && o.mode != CSAW_MODE_BATCHED) return CSAW_ERR_INVALID_ARG;
   if (o.sched < CSAW_SCHED_HEURISTIC || o.sched > CSAW_SCHED_RANDOM) return CSAW_ERR_INVALID_ARG;
   if (o.sched == CSAW_SCHED_FORCED && (o.variant < 0 || o.variant >= CSAW_NUM_VARIANTS)) return CSAW_ERR_INVALID_ARG;
+  CSAW_TRY(validate_tuning(o.tuning));
   for (int i = 0; i < n_req; ++i) {
     int st = validate_request(g, reqs[i]);
     if (st != CSAW_OK) {

@@ -59,6 +59,9 @@ struct HopLog {
 };
 
 void apply_tuning(LaunchCfg& L, const csaw_tuning* t);
+// CSAW_ERR_INVALID_ARG for shapes out of range (workgroups per CU 0..64,
+// chunk_min a power of two in 8..1024, group_min 4..32; 0 keeps a default).
+int validate_tuning(const csaw_tuning* t);
 int mlp_upload_bf16(const csaw_mlp* m, uint16_t* dst);
 
 }  // namespace csaw

@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256) void k_count(GraphDev G, const ReqDev* __restr
       // chunk descriptors: each non-empty bucket in pieces of at most cmax
       // entries (more, smaller work items when the hop is small); a piece's
       // sums are its share of the bucket's, its maxima the bucket's
-      const uint32_t cmax = chunk_max_for(n, H.expand_waves);
+      const uint32_t cmax = chunk_max_for(n, H.expand_waves, H.chunk_min);
       uint32_t nck = tid < kNumBuckets ? (hv + cmax - 1) / cmax : 0u;
       const uint32_t cincl = wave_incl_scan_u32(nck);
       __syncthreads();
@@ -544,7 +544,7 @@ __global__ __launch_bounds__(256) void k_expand_wave(GraphDev G, csaw_graph_view
     if (!cc.valid) break;
     const uint64_t t0 = wall_clock64();
     const Chunk c = H.chunks[cc.ci];
-    const int gs = group_size_for(c.max_f);
+    const int gs = group_size_for(c.max_f, H.group_min);
     for (int k0 = 0; k0 < c.len; k0 += kWave) {
       const int64_t first = (int64_t)c.start + k0;
       const int cnt = min(kWave, (int)c.len - k0);
@@ -588,7 +588,7 @@ __global__ __launch_bounds__(256) void k_expand_block(GraphDev G, csaw_graph_vie
     const uint64_t t0 = wall_clock64();
     const Chunk c = H.chunks[cc.ci];
     if (G32 || groups_ok) {
-      const int gs = group_size_for(c.max_f);
+      const int gs = group_size_for(c.max_f, H.group_min);
       for (int k0 = wid * kWave; k0 < c.len; k0 += 4 * kWave) {
         const int64_t first = (int64_t)c.start + k0;
         const int cnt = min(kWave, (int)c.len - k0);
@@ -652,7 +652,7 @@ __global__ __launch_bounds__(256) void k_expand_copy(GraphDev G, csaw_graph_view
     const ChunkCursor cc = next_chunk_wave(A, kQueueCopy);
     if (!cc.valid) break;
     const Chunk c = H.chunks[cc.ci];
-    const int gs = group_size_for(c.max_f);
+    const int gs = group_size_for(c.max_f, H.group_min);
     for (int k0 = 0; k0 < c.len; k0 += kWave)
       group_expand_batch<kUniform, kKeys32>(G, hv, R, gs2req, H, H.order, (int64_t)c.start + k0,
                                             min(kWave, (int)c.len - k0), gs, nullptr);

@@ -45,6 +45,8 @@ struct LaunchCfg {
   int thread_per_cu = 4;   // THREAD workgroups per CU
   int block_per_cu = 2;    // BLOCK workgroups per CU
   int grid_override = 0;
+  int chunk_min = kChunkMin;  // tile shapes: smallest chunk piece,
+  int group_min = kGroupMin;  // smallest sub-wave group
 };
 
 struct ExpandArgs {

@@ -406,6 +406,7 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
   if (o.mode != CSAW_MODE_PLAIN && o.mode != CSAW_MODE_BATCHED) return CSAW_ERR_INVALID_ARG;
   if (o.slice_policy < CSAW_SLICE_AUTO || o.slice_policy > CSAW_SLICE_DIRECT) return CSAW_ERR_INVALID_ARG;
   if (o.sched < CSAW_SCHED_HEURISTIC || o.sched > CSAW_SCHED_RANDOM) return CSAW_ERR_INVALID_ARG;
+  CSAW_TRY(validate_tuning(o.tuning));
   if (o.sched == CSAW_SCHED_FORCED && (o.variant < 0 || o.variant >= CSAW_NUM_VARIANTS)) return CSAW_ERR_INVALID_ARG;
   if (o.mode == CSAW_MODE_BATCHED && o.sched == CSAW_SCHED_MLP && !o.mlp) {
     set_last_error("CSAW_SCHED_MLP needs a trained csaw_mlp");
@@ -679,6 +680,8 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
         H.chunk_cap = chunk_cap;
         H.flags = d_ctr + kCtrFlags;
         H.expand_waves = expand_waves;
+        H.chunk_min = L.chunk_min;
+        H.group_min = L.group_min;
       }
       CSAW_TRY(launch_count(L, G, d_req, d_gs2req, H, false, nmax[h]));
       CSAW_TRY(launch_scans(L, H, d_tot + h, nmax[h]));

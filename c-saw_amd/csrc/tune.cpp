@@ -15,6 +15,20 @@ void apply_tuning(LaunchCfg& L, const csaw_tuning* t) {
   if (t->thread_per_cu > 0) L.thread_per_cu = t->thread_per_cu;
   if (t->block_per_cu > 0) L.block_per_cu = t->block_per_cu;
   if (t->grid_override > 0) L.grid_override = t->grid_override;
+  if (t->chunk_min > 0) L.chunk_min = t->chunk_min;
+  if (t->group_min > 0) L.group_min = t->group_min;
+}
+
+int validate_tuning(const csaw_tuning* t) {
+  if (!t) return CSAW_OK;
+  const int32_t cu[] = {t->count_per_cu, t->expand_per_cu, t->thread_per_cu, t->block_per_cu};
+  for (int32_t v : cu)
+    if (v < 0 || v > 64) return CSAW_ERR_INVALID_ARG;
+  if (t->grid_override < 0) return CSAW_ERR_INVALID_ARG;
+  const int32_t c = t->chunk_min;
+  if (c != 0 && (c < kChunkMin || c > kBlockEntries || (c & (c - 1)) != 0)) return CSAW_ERR_INVALID_ARG;
+  if (t->group_min != 0 && (t->group_min < kGroupMin || t->group_min > 32)) return CSAW_ERR_INVALID_ARG;
+  return CSAW_OK;
 }
 
 }  // namespace csaw
@@ -45,7 +59,8 @@ extern "C" int csaw_plan_chunk_log(const csaw_plan* p, float* features, int32_t*
   return CSAW_OK;
 }
 
-// Tries workgroups-per-CU settings for each kernel family in turn
+// Tries workgroups-per-CU settings for each kernel family, then the tile
+// shapes (smallest chunk, smallest sub-wave group), one knob at a time
 // (coordinate descent), timing full runs of the plan with hipEvents.
 extern "C" int csaw_autotune(csaw_graph* g, const csaw_request* reqs, int32_t n_req, const csaw_plan_opts* base,
                              csaw_tuning* best, double* best_ms) {
@@ -62,6 +77,8 @@ extern "C" int csaw_autotune(csaw_graph* g, const csaw_request* reqs, int32_t n_
   cur.expand_per_cu = def.expand_per_cu;
   cur.thread_per_cu = def.thread_per_cu;
   cur.block_per_cu = def.block_per_cu;
+  cur.chunk_min = def.chunk_min;
+  cur.group_min = def.group_min;
   auto measure = [&](const csaw_tuning& t, double* ms) -> int {
     o.tuning = &t;
     csaw_plan* p = nullptr;
@@ -79,12 +96,17 @@ extern "C" int csaw_autotune(csaw_graph* g, const csaw_request* reqs, int32_t n_
   };
   double bm = 0;
   CSAW_TRY(measure(cur, &bm));
-  const int cand[] = {1, 2, 4, 8, 16};
-  int32_t* fields[] = {&cur.expand_per_cu, &cur.thread_per_cu, &cur.block_per_cu, &cur.count_per_cu};
-  for (int32_t* f : fields) {
+  const std::vector<int> per_cu = {1, 2, 4, 8, 16};
+  const std::vector<int> chunk = {8, 16, 32, 64, 128, 256};
+  const std::vector<int> group = {4, 8, 16};
+  const std::pair<int32_t*, const std::vector<int>*> knobs[] = {
+      {&cur.expand_per_cu, &per_cu}, {&cur.thread_per_cu, &per_cu}, {&cur.block_per_cu, &per_cu},
+      {&cur.count_per_cu, &per_cu},  {&cur.chunk_min, &chunk},       {&cur.group_min, &group}};
+  for (const auto& kn : knobs) {
+    int32_t* f = kn.first;
     const int32_t keep = *f;
     int32_t bv = keep;
-    for (int c : cand) {
+    for (int c : *kn.second) {
       if (c == keep) continue;
       *f = c;
       double ms = 0;

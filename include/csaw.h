@@ -170,6 +170,13 @@ typedef struct csaw_tuning {
   int32_t thread_per_cu;
   int32_t block_per_cu;
   int32_t grid_override; /* fixed grid for the plain expand kernel (0: auto) */
+  /* Tile shapes (0: default).  chunk_min: smallest chunk of frontier
+   * entries a queue hands out (a power of two, 8 .. 1024; chunks grow to
+   * whole 1024-entry buckets when the hop is large).  group_min: smallest
+   * sub-wave group of lanes that expands one entry (4 .. 32; a group is at
+   * least the entry's fanout).  The edges do not depend on either. */
+  int32_t chunk_min;
+  int32_t group_min;
 } csaw_tuning;
 
 /* Learned cost model (16 -> 64 -> 64 -> CSAW_NUM_VARIANTS, BF16 weights,
@@ -269,7 +276,9 @@ int csaw_plan_destroy(csaw_plan *p);
  * is stable across runs of the same plan (hop << 32 | block*128 + bucket). */
 int csaw_plan_chunk_log(const csaw_plan *p, float *features, int32_t *variant, float *log2_ticks,
                         uint64_t *keys, int64_t max_n, int64_t *n);
-/* Tries launch shapes on the given requests and returns the fastest. */
+/* Tries launch and tile shapes (coordinate descent over workgroups per CU
+ * of each kernel family, chunk_min and group_min) on the given requests and
+ * returns the fastest; best_ms is its mean device time per run. */
 int csaw_autotune(csaw_graph *g, const csaw_request *reqs, int32_t n_req, const csaw_plan_opts *base,
                   csaw_tuning *best, double *best_ms);
 

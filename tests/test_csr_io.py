@@ -95,3 +95,33 @@ def test_write_bin_round_trip(C, tmp_path):
     assert os.path.getsize(beg) == 8 * 4 and os.path.getsize(csr) == 8 * 5
     with pytest.raises(C.CsawError):
         C.write_bin(beg, csr, off, np.array([1, 2, 0, 1, 3], np.int32))  # id 3 >= V
+
+
+def test_load_bin_rejects_malformed_files(C, tmp_path):
+    # every check here runs on the host before anything is uploaded
+    beg, csr, wf = str(tmp_path / "beg.bin"), str(tmp_path / "csr.bin"), str(tmp_path / "w.bin")
+
+    def files(off, ids, nbytes_beg=None, w=None):
+        b = np.asarray(off, np.int64).tobytes()
+        open(beg, "wb").write(b[:nbytes_beg] if nbytes_beg is not None else b)
+        np.asarray(ids, np.int64).tofile(csr)
+        if w is not None:
+            np.asarray(w, np.float32).tofile(wf)
+
+    files([0, 1, 2], [1, 0], nbytes_beg=20)  # not a multiple of 8 bytes
+    with pytest.raises(C.CsawError):
+        C.Graph.load_bin(beg, csr)
+    files([0, 1, 3], [1, 0])  # beg[V] = 3 ids, csr.bin holds 2
+    with pytest.raises(C.CsawError):
+        C.Graph.load_bin(beg, csr)
+    files([0, 1, 2], [1, 2])  # id 2 outside [0, V = 2)
+    with pytest.raises(C.CsawError):
+        C.Graph.load_bin(beg, csr)
+    files([0, 1, 2], [1, -1])
+    with pytest.raises(C.CsawError):
+        C.Graph.load_bin(beg, csr)
+    files([0, 1, 2], [1, 0], w=[0.5])  # fewer than E weights
+    with pytest.raises(C.CsawError):
+        C.Graph.load_bin(beg, csr, wf)
+    with pytest.raises(C.CsawError):
+        C.Graph.load_bin(str(tmp_path / "missing.bin"), csr)

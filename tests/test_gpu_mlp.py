@@ -71,6 +71,8 @@ def test_learned_scheduler_beats_heuristic(C):
     print("held-out regret", r)
     assert r["chunks"] > 100
     assert r["mlp"] < r["heuristic"], r
+    # and the best variant forced for every chunk (the best fixed policy)
+    assert r["mlp"] < min(r["single_variant"]), r
 
 
 def test_learned_scheduler_without_cdf_cache(C):
