@@ -462,6 +462,8 @@ _sig("csaw_stream_sample", [P, P, i64, i32, P, P])
 _sig("csaw_stream_sample_opts", [P, P, i64, i32, P, P, P])
 _sig("csaw_edges_from_host", [i32, i32, i64, P, P, P, P, P])
 _sig("csaw_mdrw", [P, u64, u32, i32, i32, i32, P, P, P, P, P])
+# device pointers (seeds, src, dst, offsets), stream, sync; allocates nothing
+_sig("csaw_mdrw_noalloc", [P, u64, u32, i32, i32, i32, P, P, P, P, P, ctypes.c_int])
 
 
 class HostGraph:

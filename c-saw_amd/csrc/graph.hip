@@ -255,6 +255,7 @@ int graph_alloc_common(csaw_graph* g) {
   CSAW_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
   const char* keys = getenv("CSAW_INT_CDF_KEYS");
   g->wide_int_cdf = keys && strcmp(keys, "64") == 0;
+  CSAW_TRY(mdrw_scratch_setup());
   return CSAW_OK;
 }
 

@@ -64,5 +64,8 @@ int hub_cdf_segments(const GraphDev& G, const csaw_graph_view& hv, int kind, int
 int rmat_generate_device(int scale, int edge_factor, double a, double b, double c, uint64_t key, hipStream_t stream,
                          int num_cu, int64_t* V, int64_t* E, int64_t** d_off, int32_t** d_adj);
 int graph_alloc_common(csaw_graph* g);
+// Looks up this device's address of the allocation-free MDRW scratch (once
+// per device; called with the device current when a graph is created).
+int mdrw_scratch_setup();
 int device_init(int* num_cu);
 }  // namespace csaw

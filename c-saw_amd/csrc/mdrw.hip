@@ -11,6 +11,7 @@
 // of the grid commits), every sample is walked.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "graph.hpp"
@@ -20,15 +21,31 @@ namespace csaw {
 
 constexpr int kMdrwMaxFs = 4096;
 
+// bad: null when the host has validated the seeds; otherwise a sample with
+// a seed outside [0, V) emits nothing and sets *bad (seeds read on the device).
 __global__ __launch_bounds__(64) void k_mdrw(GraphDev G, csaw_graph_view hv, uint32_t k0, uint32_t k1, uint32_t request,
                                             int32_t S, int32_t fs, int32_t depth, const int32_t* __restrict__ seeds,
                                             int32_t* __restrict__ src, int32_t* __restrict__ dst,
-                                            int32_t* __restrict__ cnt) {
+                                            int32_t* __restrict__ cnt, uint32_t* __restrict__ bad) {
   __shared__ int32_t front[kMdrwMaxFs];
   __shared__ uint32_t wts[kMdrwMaxFs];
   const int lane = lane_id();
   const uint32_t c3 = (kDomMdrw << 28) | request;
   for (int32_t s = blockIdx.x; s < S; s += gridDim.x) {
+    if (bad) {
+      bool out = false;
+      for (int q = lane; q < fs; q += kWave) {
+        const int32_t v = seeds[(int64_t)s * fs + q];
+        out |= v < 0 || (int64_t)v >= G.V;
+      }
+      if (ballot(out)) {
+        if (lane == 0) {
+          cnt[s] = 0;
+          atomicOr(bad, 1u);
+        }
+        continue;
+      }
+    }
     uint64_t T = 0;
     for (int q0 = 0; q0 < fs; q0 += kWave) {
       const int q = q0 + lane;
@@ -82,6 +99,90 @@ __global__ __launch_bounds__(64) void k_mdrw(GraphDev G, csaw_graph_view hv, uin
   }
 }
 
+// Per-sample edge offsets from the per-sample counts: one workgroup of 1024
+// threads, each summing a contiguous run of samples, then a block scan.
+__global__ __launch_bounds__(1024) void k_mdrw_offsets(const int32_t* __restrict__ cnt, int32_t S,
+                                                       int64_t* __restrict__ offsets) {
+  __shared__ uint64_t wsum[16];
+  const int t = threadIdx.x, lane = lane_id(), wid = t / kWave;
+  const int64_t per = ((int64_t)S + 1023) / 1024;
+  const int64_t b = (int64_t)t * per, e = min((int64_t)S, b + per);
+  uint64_t loc = 0;
+  for (int64_t i = b; i < e; ++i) loc += (uint64_t)cnt[i];
+  const uint64_t inc = wave_incl_scan_u64(loc);
+  if (lane == kWave - 1) wsum[wid] = inc;
+  __syncthreads();
+  if (wid == 0) {
+    const uint64_t x = lane < 16 ? wsum[lane] : 0;
+    const uint64_t xi = wave_incl_scan_u64(x);
+    if (lane < 16) wsum[lane] = xi - x;
+  }
+  __syncthreads();
+  uint64_t run = wsum[wid] + inc - loc;
+  for (int64_t i = b; i < e; ++i) {
+    offsets[i] = (int64_t)run;
+    run += (uint64_t)cnt[i];
+  }
+  if (t == 1023) offsets[S] = (int64_t)run;  // the last thread's prefix + its run = the total
+}
+
+// Moves each sample's edges from its depth-strided slots to its offset: one
+// wave per sample.
+__global__ __launch_bounds__(256) void k_mdrw_compact(const int32_t* __restrict__ cnt, const int64_t* __restrict__ offsets,
+                                                      const int32_t* __restrict__ ssrc, const int32_t* __restrict__ sdst,
+                                                      int32_t S, int32_t depth, int32_t* __restrict__ src,
+                                                      int32_t* __restrict__ dst) {
+  const int lane = lane_id();
+  const int64_t nw = (int64_t)gridDim.x * 4;
+  for (int64_t s = (int64_t)blockIdx.x * 4 + threadIdx.x / kWave; s < S; s += nw) {
+    const int32_t c = cnt[s];
+    const int64_t o = offsets[s];
+    for (int32_t i = lane; i < c; i += kWave) {
+      src[o + i] = ssrc[s * depth + i];
+      dst[o + i] = sdst[s * depth + i];
+    }
+  }
+}
+
+__global__ void k_mdrw_clear(uint32_t* __restrict__ p) {
+  if (threadIdx.x == 0) *p = 0u;
+}
+
+// ------------------------------------------------------ allocation-free MDRW
+// Device scratch of csaw_mdrw_noalloc: a flag word, then per-sample counts and
+// the depth-strided edge slots.  Each GPU holds its own copy for the life of
+// the process; host code reaches it only through the address that
+// hipGetSymbolAddress returned on that device (mdrw_scratch_ptr).
+constexpr int64_t kMdrwScratchBytes = 32ll << 20;
+static __device__ __align__(256) unsigned char scratch[kMdrwScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int mdrw_scratch_setup() {
+  int dev = 0;
+  CSAW_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_last_error("device id beyond the scratch table");
+    return CSAW_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return CSAW_OK;
+  void* p = nullptr;
+  CSAW_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+  return CSAW_OK;
+}
+
+// The scratch bytes one call needs (flag, counts, two slot arrays).
+static int64_t mdrw_scratch_need(int64_t samples, int64_t depth) {
+  const auto up = [](int64_t b) { return (b + 255) & ~(int64_t)255; };
+  return 256 + up(4 * samples) + 2 * up(4 * samples * depth);
+}
+
 }  // namespace csaw
 
 using namespace csaw;
@@ -118,7 +219,7 @@ extern "C" int csaw_mdrw(csaw_graph* g, uint64_t seed, uint32_t request_id, int3
     if (samples > 0) {
       const int grid = std::min<int>(samples, g->num_cu * 16);
       k_mdrw<<<grid, 64, 0, g->stream>>>(g->view(), g->hook_view(), (uint32_t)seed, (uint32_t)(seed >> 32), request_id,
-                                         samples, frontier_size, depth, d_seeds, d_src, d_dst, d_cnt);
+                                         samples, frontier_size, depth, d_seeds, d_src, d_dst, d_cnt, nullptr);
       CSAW_LAUNCH_CHECK();
     }
     CSAW_HIP(hipEventRecord(e1, g->stream));
@@ -152,4 +253,58 @@ extern "C" int csaw_mdrw(csaw_graph* g, uint64_t seed, uint32_t request_id, int3
   int st = body();
   cleanup();
   return st;
+}
+
+extern "C" int csaw_mdrw_noalloc(csaw_graph* g, uint64_t seed, uint32_t request_id, int32_t samples,
+                                 int32_t frontier_size, int32_t depth, const int32_t* d_seeds, int32_t* d_src,
+                                 int32_t* d_dst, int64_t* d_offsets, void* stream, int sync) {
+  if (!g || samples < 0 || frontier_size < 1 || frontier_size > kMdrwMaxFs || depth < 0 || !d_offsets ||
+      request_id >= (1u << 28) || (samples > 0 && (!d_seeds || (depth > 0 && (!d_src || !d_dst)))))
+    return CSAW_ERR_INVALID_ARG;
+  const int64_t need = mdrw_scratch_need(samples, depth);
+  if (need > kMdrwScratchBytes) {
+    set_last_error("csaw_mdrw_noalloc: samples x depth = " + std::to_string((int64_t)samples * depth) +
+                   " needs " + std::to_string(need) + " bytes of scratch; the maximum is " +
+                   std::to_string(kMdrwScratchBytes) + " (about samples * (8 * depth + 4))");
+    return CSAW_ERR_CAPACITY;
+  }
+  int dev = 0;
+  CSAW_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return CSAW_ERR_UNSUPPORTED;
+  unsigned char* base = g_scratch_ptr[dev];
+  if (!base) {
+    CSAW_TRY(mdrw_scratch_setup());
+    base = g_scratch_ptr[dev];
+  }
+  const auto up = [](int64_t b) { return (b + 255) & ~(int64_t)255; };
+  uint32_t* bad = reinterpret_cast<uint32_t*>(base);
+  int32_t* cnt = reinterpret_cast<int32_t*>(base + 256);
+  int32_t* ssrc = reinterpret_cast<int32_t*>(base + 256 + up(4 * (int64_t)samples));
+  int32_t* sdst = reinterpret_cast<int32_t*>(base + 256 + up(4 * (int64_t)samples) + up(4 * (int64_t)samples * depth));
+  hipStream_t st = stream ? (hipStream_t)stream : g->stream;
+  k_mdrw_clear<<<1, 64, 0, st>>>(bad);
+  CSAW_LAUNCH_CHECK();
+  if (samples > 0) {
+    const int grid = std::min<int>(samples, g->num_cu * 16);
+    k_mdrw<<<grid, 64, 0, st>>>(g->view(), g->hook_view(), (uint32_t)seed, (uint32_t)(seed >> 32), request_id, samples,
+                                frontier_size, depth, d_seeds, ssrc, sdst, cnt, bad);
+    CSAW_LAUNCH_CHECK();
+  }
+  k_mdrw_offsets<<<1, 1024, 0, st>>>(cnt, samples, d_offsets);
+  CSAW_LAUNCH_CHECK();
+  if (samples > 0 && depth > 0) {
+    const int grid = (int)std::min<int64_t>(((int64_t)samples + 3) / 4, (int64_t)g->num_cu * 8);
+    k_mdrw_compact<<<grid, 256, 0, st>>>(cnt, d_offsets, ssrc, sdst, samples, depth, d_src, d_dst);
+    CSAW_LAUNCH_CHECK();
+  }
+  if (sync) {
+    uint32_t flag = 0;
+    CSAW_HIP(hipMemcpyAsync(&flag, bad, sizeof(flag), hipMemcpyDeviceToHost, st));
+    CSAW_HIP(hipStreamSynchronize(st));
+    if (flag) {
+      set_last_error("csaw_mdrw_noalloc: a seed is outside [0, V) (its sample emitted nothing)");
+      return CSAW_ERR_INVALID_ARG;
+    }
+  }
+  return CSAW_OK;
 }

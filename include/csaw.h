@@ -205,6 +205,20 @@ int csaw_mdrw(csaw_graph *g, uint64_t seed, uint32_t request_id, int32_t samples
               int32_t depth, const int32_t *seeds, int32_t *src, int32_t *dst, int64_t *offsets,
               csaw_stats *stats);
 
+/* csaw_mdrw over caller-owned DEVICE buffers that allocates no memory: no
+ * counterpart in the reference.  For callers that cannot allocate (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * d_seeds: samples*frontier_size ids; d_src, d_dst: samples*depth entries;
+ * d_offsets: samples+1 (the edges are compacted as csaw_mdrw returns them).
+ * Work is enqueued on `stream` (NULL: the graph's stream).  Scratch is a
+ * static 32 MiB device array per GPU, which limits samples*(8*depth+4) to
+ * about 32 MiB (CSAW_ERR_CAPACITY beyond).  Calls that overlap on one GPU
+ * race on that scratch.  sync != 0 waits and reports a seed outside [0, V)
+ * (CSAW_ERR_INVALID_ARG); such a seed's sample emits nothing in any case. */
+int csaw_mdrw_noalloc(csaw_graph *g, uint64_t seed, uint32_t request_id, int32_t samples,
+                      int32_t frontier_size, int32_t depth, const int32_t *d_seeds, int32_t *d_src,
+                      int32_t *d_dst, int64_t *d_offsets, void *stream, int sync);
+
 /* Reusable walk plan: buffers allocated once, seeds resident on the device. */
 typedef struct csaw_walk_plan csaw_walk_plan;
 int csaw_walk_plan_create(csaw_graph *g, int32_t bias, uint64_t seed, uint32_t request_id,
