@@ -393,11 +393,18 @@ __global__ __launch_bounds__(256) void k_choose(ChooseArgs A, HopDev H) {
   for (uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x; ci - threadIdx.x < nc; ci += gridDim.x * blockDim.x) {
     const bool act = ci < nc;
     int q = -1;
+    // algorithmic bytes of the chunk for the kernel kind that will run it
+    // (timing runs only), summed per wave before one atomic per kind
+    unsigned long long cb = 0;
+    int ck = -1;
     if (act) {
       const Chunk c = H.chunks[ci];
       if (c.bucket == kBucketCopy) {
         q = kQueueCopy;
-        if (H.bytes) atomicAdd(&H.bytes[kKernCopy], (unsigned long long)expansion_bytes(0, false, c.len, (int64_t)c.sum_d, 0, 0, true));
+        if (H.bytes) {
+          ck = kKernCopy;
+          cb = (unsigned long long)expansion_bytes(0, false, c.len, (int64_t)c.sum_d, 0, 0, true);
+        }
       } else {
         const int kind = bucket_kind(c.bucket);
         int var = heuristic_variant(kind, c, A.int_cache);
@@ -421,8 +428,18 @@ __global__ __launch_bounds__(256) void k_choose(ChooseArgs A, HopDev H) {
           const int kk = var == CSAW_VAR_THREAD ? kKernThread : (var <= CSAW_VAR_WAVE_HASH ? kKernWave : kKernBlock);
           const int cap_l = var == CSAW_VAR_THREAD ? 5 : (var <= CSAW_VAR_WAVE_HASH ? 10 : 13);
           const bool built = kind == kInt && !A.int_cache && class_max_log(bucket_class(c.bucket)) <= cap_l;
-          atomicAdd(&H.bytes[kk], (unsigned long long)expansion_bytes(kind, built, c.len, (int64_t)c.sum_d, c.sum_f, l, false));
+          ck = kk;
+          cb = (unsigned long long)expansion_bytes(kind, built, c.len, (int64_t)c.sum_d, c.sum_f, l, false);
         }
+      }
+    }
+    if (H.bytes) {
+      const int kinds[4] = {kKernCopy, kKernThread, kKernWave, kKernBlock};
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        unsigned long long x = ck == kinds[t] ? cb : 0ull;
+        for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+        if (lane_id() == 0 && x) atomicAdd(&H.bytes[kinds[t]], x);
       }
     }
     // wave-aggregated append to queue q
