@@ -49,10 +49,10 @@ constexpr int kBlockBitmapCap = 65536;
 constexpr int kNumBuckets = 128;     // 0 = copy-all; 1 + bias*32 + ceil(log2 d)
 constexpr int kBucketCopy = 0;
 // Chunk descriptors of a hop: a block's buckets (at most 1 + 3 kinds x 8
-// degree classes non-empty) split into pieces of cmax entries give at most
-// 25 + 1024 / cmax per block, and cmax >= n / (expand waves), so the hop has
-// at most 26 * blocks + waves chunks.
-constexpr int kChunksPerBlock = 26;
+// degree classes + 3 region-remap buckets non-empty) split into pieces of
+// cmax entries give at most 28 + 1024 / cmax per block, and cmax >= n /
+// (expand waves), so the hop has at most 28 * blocks + waves chunks.
+constexpr int kChunksPerBlock = 28;
 constexpr int kChunkMin = 8;         // smallest chunk piece (buckets split for small hops)
 
 // Bias kinds inside kernels: integer biases share one code path.
@@ -191,17 +191,26 @@ __host__ __device__ __forceinline__ int class_max_log(int cls) {
 }
 
 // bucket id: 0 copy-all, else 1 + kDegClasses*kind + degree class (kind 0..2)
+// Entries of region-remap requests (dedup == 2) that select (d > F) get
+// one bucket per bias kind, expanded by their own whole-wave kernel.
+constexpr int kBucketRemap0 = 1 + 3 * kDegClasses;
+
 __host__ __device__ __forceinline__ int entry_bucket(int64_t d, int32_t F, int32_t dedup, int32_t bias) {
   if (dedup && d <= F) return kBucketCopy;
   int kind = bias_kind(bias);
+  if (dedup == 2) return kBucketRemap0 + kind;
   int l = ceil_log2_u32((uint32_t)(d > 0x7fffffff ? 0x7fffffff : d));
   // a uniform draw costs the same at any degree: degrees up to 1024 share
   // one class, so a block's uniform entries form few, full chunks
   if (kind == kUniform && l <= 10) l = 10;
   return 1 + kDegClasses * kind + degree_class(l);
 }
-__host__ __device__ __forceinline__ int bucket_kind(int bucket) { return (bucket - 1) / kDegClasses; }
-__host__ __device__ __forceinline__ int bucket_class(int bucket) { return (bucket - 1) % kDegClasses; }
+__host__ __device__ __forceinline__ int bucket_kind(int bucket) {
+  return bucket >= kBucketRemap0 ? bucket - kBucketRemap0 : (bucket - 1) / kDegClasses;
+}
+__host__ __device__ __forceinline__ int bucket_class(int bucket) {
+  return bucket >= kBucketRemap0 ? kDegClasses - 1 : (bucket - 1) % kDegClasses;
+}
 
 // Algorithmic bytes of one expansion (each datum counted once per use, no
 // cache-line amplification): frontier entry 8 + offset 8 + degree 4 = 20;

@@ -70,6 +70,7 @@ using namespace csaw;
 static int validate_request(const csaw_graph* g, const csaw_request& r) {
   if (r.samples < 0 || r.depth < 0 || r.depth > CSAW_MAX_HOPS) return CSAW_ERR_INVALID_ARG;
   if (r.bias < CSAW_BIAS_UNIFORM || r.bias > CSAW_BIAS_USER) return CSAW_ERR_INVALID_ARG;
+  if (r.dedup < 0 || r.dedup > 2) return CSAW_ERR_INVALID_ARG;
   if (r.bias == CSAW_BIAS_WEIGHT && !g->w) return CSAW_ERR_INVALID_ARG;
   if (r.request_id >= (1u << 20)) return CSAW_ERR_INVALID_ARG;
   for (int h = 0; h < r.depth; ++h)
@@ -122,6 +123,7 @@ static int build_unit(csaw_plan* p, Unit& u) {
   u.G = 0;
   u.depth_max = 0;
   u.kinds[0] = u.kinds[1] = u.kinds[2] = false;
+  u.remap = false;
   std::vector<ReqDev> rd;
   for (int ri : u.reqs) {
     const csaw_request& r = p->reqs[ri];
@@ -130,7 +132,7 @@ static int build_unit(csaw_plan* p, Unit& u) {
     for (int h = 0; h < CSAW_MAX_HOPS; ++h) d.fanout[h] = h < r.depth ? r.fanouts[h] : 0;
     d.depth = r.depth;
     d.bias = r.bias;
-    d.dedup = r.dedup ? 1 : 0;
+    d.dedup = r.dedup == 2 ? 2 : (r.dedup ? 1 : 0);
     d.samples = r.samples;
     d.key_lo = (uint32_t)r.seed;
     d.key_hi = (uint32_t)(r.seed >> 32);
@@ -141,6 +143,7 @@ static int build_unit(csaw_plan* p, Unit& u) {
     u.G += r.samples;
     u.depth_max = std::max(u.depth_max, r.depth);
     if (r.depth > 0 && r.samples > 0) u.kinds[bias_kind(r.bias)] = true;
+    if (r.dedup == 2) u.remap = true;
   }
   if (u.G > 0x7fffffffLL) return CSAW_ERR_CAPACITY;
   // per hop: first inactive global sample, edge capacity, max frontier entries
@@ -280,6 +283,7 @@ static int run_unit(csaw_plan* p, Unit& u, hipStream_t st, TimingHook* th) {
       E.qhead = p->counters + kCtrQhead;
       E.log = p->opts.log_chunks ? p->log : nullptr;
       E.int_cache = p->opts.cdf_cache != 0;
+      E.remap = u.remap ? 1 : 0;
       if (p->side[0]) {
         // the hop's expansion kernels (one per variant queue) run side by
         // side on four streams, so one kernel's last chunks overlap the
@@ -313,7 +317,8 @@ static int run_unit(csaw_plan* p, Unit& u, hipStream_t st, TimingHook* th) {
       int fmax = 0;
       for (int ri : u.reqs)
         if (p->reqs[ri].depth > h) fmax = std::max(fmax, p->reqs[ri].fanouts[h]);
-      const int gs = group_size_for(fmax, L.group_min);
+      // region-remap requests resolve their slots with whole waves (gs = 0)
+      const int gs = u.remap ? 0 : group_size_for(fmax, L.group_min);
       if (th) th->begin(kKernPlain, st);
       CSAW_TRY(launch_expand_plain(L, kind, G, hv, u.d_req, u.d_gs2req, H, u.nmax[h], gs));
       if (th) th->end(kKernPlain, st);

@@ -40,6 +40,7 @@ struct Unit {
   int64_t G = 0;
   int depth_max = 0;
   bool kinds[3] = {false, false, false};
+  bool remap = false;           // some request uses the region-remap mode
   std::vector<int32_t> gb;      // per hop: first inactive global sample
   std::vector<int64_t> cap;     // per hop: edge capacity
   std::vector<int64_t> nmax;    // per hop: frontier entries upper bound

@@ -408,7 +408,9 @@ __global__ __launch_bounds__(256) void k_choose(ChooseArgs A, HopDev H) {
       } else {
         const int kind = bucket_kind(c.bucket);
         int var = heuristic_variant(kind, c, A.int_cache);
-        if (A.sched == CSAW_SCHED_FORCED) {
+        if (c.bucket >= kBucketRemap0) {
+          var = CSAW_VAR_WAVE_BITMAP;  // region-remap entries: one wave each, their own queue
+        } else if (A.sched == CSAW_SCHED_FORCED) {
           if (variant_valid(A.variant, kind, c, A.int_cache)) var = A.variant;
         } else if (A.sched == CSAW_SCHED_RANDOM) {
           const U4 o = philox4x32_10(c.key, (uint32_t)c.hop, 0u, kDomExplore << 28, (uint32_t)A.explore_key,
@@ -420,7 +422,7 @@ __global__ __launch_bounds__(256) void k_choose(ChooseArgs A, HopDev H) {
         } else if (use_mlp) {
           var = mlp_choose(A, wl, c, kind, H);
         }
-        q = var * 3 + kind;
+        q = c.bucket >= kBucketRemap0 ? kQueueRemap0 + kind : var * 3 + kind;
         if (A.chosen) A.chosen[ci] = (uint8_t)var;
         if (H.bytes) {
           // mean probes per draw: the chunk's mean ceil(log2 d)
@@ -576,6 +578,28 @@ __global__ __launch_bounds__(256) void k_expand_wave(GraphDev G, csaw_graph_view
       }
     }
     log_chunk_wave(A, H, cc.ci, var, t0);
+  }
+}
+
+// Region-remap entries (dedup == 2, heur_normalize's collision policy): one
+// wave per entry, whole-wave selection (wave_select_remap); the integer CDF
+// is built in LDS (d <= kWaveIntCap) or read from the hub cache or the CDF
+// cache, the taken set is an LDS bitmap.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_expand_remap(GraphDev G, csaw_graph_view hv, const ReqDev* __restrict__ R,
+                                                      const int32_t* __restrict__ gs2req, HopDev H, ExpandArgs A) {
+  __shared__ uint64_t cdf[KIND == kInt ? 4 * kWaveIntCap : 1];
+  __shared__ uint32_t bms[4 * (kWaveBitmapCap / 32)];
+  const int wid = threadIdx.x / kWave;
+  for (;;) {
+    const ChunkCursor cc = next_chunk_wave(A, kQueueRemap0 + KIND);
+    if (!cc.valid) break;
+    const uint64_t t0 = wall_clock64();
+    const Chunk c = H.chunks[cc.ci];
+    for (int k0 = 0; k0 < c.len; k0 += kWave)
+      wave_expand_batch<KIND, true>(G, hv, R, gs2req, H, H.order, (int64_t)c.start + k0, min(kWave, (int)c.len - k0),
+                                    KIND == kInt ? cdf + wid * kWaveIntCap : nullptr, bms + wid * (kWaveBitmapCap / 32));
+    log_chunk_wave(A, H, cc.ci, CSAW_VAR_WAVE_BITMAP, t0);
   }
 }
 
@@ -804,6 +828,11 @@ int launch_expand_chunks(const LaunchCfg& L, const GraphDev& G, const csaw_graph
     }
   }
   CSAW_TRY(go(kKernCopy, k_expand_copy, L.expand_per_cu));
+  if (A.remap) {
+    if (kinds_present[kUniform]) CSAW_TRY(go(kKernWave, k_expand_remap<kUniform>, L.expand_per_cu));
+    if (kinds_present[kInt]) CSAW_TRY(go(kKernWave, k_expand_remap<kInt>, L.expand_per_cu));
+    if (kinds_present[kF32]) CSAW_TRY(go(kKernWave, k_expand_remap<kF32>, L.expand_per_cu));
+  }
   return CSAW_OK;
 }
 

@@ -7,8 +7,9 @@
 
 namespace csaw {
 
-constexpr int kNumQueues = 16;  // variant * 3 + kind, plus the copy-all queue
+constexpr int kNumQueues = 19;  // variant * 3 + kind, the copy-all queue, 3 remap queues
 constexpr int kQueueCopy = 15;
+constexpr int kQueueRemap0 = 16; // + kind: region-remap entries (one wave per entry)
 
 enum KernKind : int {
   kKernCount = 0,
@@ -56,6 +57,7 @@ struct ExpandArgs {
   uint32_t* qhead;
   ChunkLog* log;
   int32_t int_cache;  // integer biases search the CDF cache (no per-entry build)
+  int32_t remap;      // some request uses the region-remap mode (its queues may be non-empty)
 };
 
 struct ChooseArgs {

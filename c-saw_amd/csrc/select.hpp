@@ -262,6 +262,100 @@ __device__ int32_t wave_select(int64_t E, const Entry& en, const Picker& pk, uin
   return res;
 }
 
+// ---------------------------------------------------------------------------
+// Region-remap mode (dedup == 2): heur_normalize's collision policy
+// (functions.cuh:743-868).  Draws live in the normalised CDF
+// p_i = fl32(fl64(C_i) / fl64(T)) (uniform: fl32((i + 1) / d)); r picks the
+// first i with p_i > r, clamped to d - 1.  Per slot, bounds lb = 0, hb = 1;
+// each attempt draws r; on a collision with the picked interval [A, B) r is
+// remapped with the reference's float sequence (A = p_{i-1}, 0 for i = 0;
+// B = p_i) and picks again; after kReselect attempts the cyclic fallback.
+// Every lane runs the same sequence (the results agree across lanes); lane
+// j keeps slot j.  Taken set: the LDS bitmap, or the resolved lanes.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float remap_p(const Picker& pk, int32_t i) {
+  switch (pk.mode) {
+    case kPickUniform:
+      return (float)((double)(i + 1) / (double)pk.d);
+    case kPickF32:
+    case kPickF32Idx:
+      return (float)((double)static_cast<const float*>(pk.C)[pk.o + i] / (double)pk.Tf);
+    case kPickInt32Idx:
+      return (float)((double)static_cast<const uint32_t*>(pk.C)[pk.o + i] / (double)pk.T);
+    default:  // uint64 CDFs: in LDS, the hub cache or the 64-bit cache
+      return (float)((double)static_cast<const uint64_t*>(pk.C)[pk.o + i] / (double)pk.T);
+  }
+}
+
+__device__ __forceinline__ int32_t remap_pick(const Picker& pk, float r) {
+  int32_t lo = 0, hi = pk.d;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (remap_p(pk, mid) > r) hi = mid; else lo = mid + 1;
+  }
+  return lo < pk.d ? lo : pk.d - 1;
+}
+
+__device__ int32_t wave_select_remap(const Entry& en, const Picker& pk, uint32_t* bm) {
+  const int lane = lane_id();
+  const int F = en.F;
+  const int32_t d = en.d;
+  int32_t res = -1;
+  if (bm) {
+    const int words = (d + 31) >> 5;
+    for (int i = lane; i < words; i += kWave) bm[i] = 0u;
+    wave_fence();
+  }
+  auto taken = [&](int32_t c, int jj) -> bool {
+    return bm ? ((bm[c >> 5] >> (c & 31)) & 1u) != 0u : ballot(res == c) != 0ull;
+  };
+  for (int jj = 0; jj < F; ++jj) {
+    float lb = 0.0f, hb = 1.0f;
+    int32_t i = 0, chosen = -1;
+    for (int a = 0; a < kReselect; ++a) {
+      float r = philox_u24(draw(en, jj, a));
+      i = remap_pick(pk, r);
+      if (!taken(i, jj)) {
+        chosen = i;
+        break;
+      }
+      const float A = i > 0 ? remap_p(pk, i - 1) : 0.0f;
+      const float B = remap_p(pk, i);
+      const float lambda = __fdiv_rn(__fsub_rn(A, lb), __fsub_rn(__fadd_rn(__fsub_rn(A, lb), hb), B));
+      const float delta = __fdiv_rn(__fsub_rn(B, A), __fsub_rn(hb, lb));
+      r = __fdiv_rn(r, lambda);
+      if (r < A) {
+        hb = A;
+      } else {
+        r = __fadd_rn(r, delta);
+        lb = B;
+      }
+      i = remap_pick(pk, r);
+      if (!taken(i, jj)) {
+        chosen = i;
+        break;
+      }
+    }
+    if (chosen < 0) {
+      for (int q = 0; q < d; q += kWave) {
+        const int32_t c = (int32_t)(((int64_t)i + 1 + q + lane) % d);
+        const bool fr = (q + lane < d) && !wave_taken_lane(c, res, jj, bm);
+        const uint64_t m = ballot(fr);
+        if (m) {
+          chosen = readlane(c, __builtin_ctzll(m));
+          break;
+        }
+      }
+    }
+    if (bm) {
+      if (lane == 0) bm[chosen >> 5] |= 1u << (chosen & 31);
+      wave_fence();
+    }
+    if (lane == jj) res = chosen;
+  }
+  return res;
+}
+
 // Builds the integer CDF of entry en in LDS with one wave (d <= cap).
 __device__ __forceinline__ uint64_t wave_build_int_cdf(const GraphDev& G, const csaw_graph_view& hv, const Entry& en,
                                                        uint64_t* C) {
@@ -302,7 +396,8 @@ __device__ void expand_entry_wave(const GraphDev& G, const csaw_graph_view& hv, 
     pk.T = wave_build_int_cdf(G, hv, en, cdf);
     pk.mode = pk.T > 0 ? kPickLds : kPickUniform;
   }
-  const int32_t res = wave_select<IDX>(G.E, en, pk, (bm && en.d <= kWaveBitmapCap) ? bm : nullptr);
+  uint32_t* tb = (bm && en.d <= kWaveBitmapCap) ? bm : nullptr;
+  const int32_t res = en.dedup == 2 ? wave_select_remap(en, pk, tb) : wave_select<IDX>(G.E, en, pk, tb);
   if (lane_id() < en.F) write_edge(G, hv, H, en, en.pos + lane_id(), res);
 }
 

@@ -406,6 +406,7 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
   if (o.mode != CSAW_MODE_PLAIN && o.mode != CSAW_MODE_BATCHED) return CSAW_ERR_INVALID_ARG;
   if (o.slice_policy < CSAW_SLICE_AUTO || o.slice_policy > CSAW_SLICE_DIRECT) return CSAW_ERR_INVALID_ARG;
   if (o.sched < CSAW_SCHED_HEURISTIC || o.sched > CSAW_SCHED_RANDOM) return CSAW_ERR_INVALID_ARG;
+  if (req->dedup < 0 || req->dedup > 2) return CSAW_ERR_INVALID_ARG;
   CSAW_TRY(validate_tuning(o.tuning));
   if (o.sched == CSAW_SCHED_FORCED && (o.variant < 0 || o.variant >= CSAW_NUM_VARIANTS)) return CSAW_ERR_INVALID_ARG;
   if (o.mode == CSAW_MODE_BATCHED && o.sched == CSAW_SCHED_MLP && !o.mlp) {
@@ -552,7 +553,7 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
     for (int h = 0; h < r.depth; ++h) rd.fanout[h] = r.fanouts[h];
     rd.depth = r.depth;
     rd.bias = r.bias;
-    rd.dedup = r.dedup ? 1 : 0;
+    rd.dedup = r.dedup == 2 ? 2 : (r.dedup ? 1 : 0);
     rd.samples = S;
     rd.key_lo = (uint32_t)r.seed;
     rd.key_hi = (uint32_t)(r.seed >> 32);
@@ -924,6 +925,7 @@ extern "C" int csaw_stream_sample_opts(csaw_host_graph* hg, const csaw_request* 
           X.qhead = d_ctr + kCtrQhead;
           X.log = d_log;
           X.int_cache = int_bias ? 1 : 0;
+          X.remap = r.dedup == 2 ? 1 : 0;
           CSAW_TRY(launch_expand_chunks(L, Gp, hvp, d_req, d_gs2req, H, X, kinds_present, nullptr));
         } else {
           CSAW_TRY(launch_expand_list(L, kind, Gp, hvp, d_req, d_gs2req, H, d_list + poff[p], (int64_t)cnt[p]));

@@ -127,7 +127,10 @@ typedef struct csaw_params {
   const int32_t *fanouts;/* optional per-hop fanouts, `depth` entries      */
   int32_t bias;          /* csaw_bias                                      */
   int32_t dedup;         /* 1: no repeated neighbour per source; 0: with   */
-                         /*  replacement                                   */
+                         /*  replacement; 2: no repeats, collisions        */
+                         /*  resolved by heur_normalize's region remap     */
+                         /*  (the reference's default, functions.cuh:743-  */
+                         /*  868) instead of redraws                       */
   uint64_t seed;         /* Philox key                                     */
   uint32_t request_id;   /* counter word; keeps batched == plain           */
   int32_t sched;         /* csaw_sched                                     */
@@ -234,7 +237,7 @@ typedef struct csaw_request {
   int32_t depth;
   int32_t fanouts[CSAW_MAX_HOPS];
   int32_t bias;
-  int32_t dedup;
+  int32_t dedup;         /* as csaw_params.dedup (0, 1 or 2)            */
   uint64_t seed;
   uint32_t request_id;
   const int32_t *seeds; /* host, `samples` entries */

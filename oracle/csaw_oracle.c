@@ -241,6 +241,27 @@ static inline int64_t its_pick(const its_t *s, const uint32_t o[4]) {
   return (int64_t)mulhi64(t, (uint64_t)s->d);
 }
 
+/* Region-remap mode (heur_normalize, functions.cuh:743-868; the reference's
+ * default collision policy, NORMALIZE at main.cu:26, 80-82).  Its draws live
+ * in the normalised CDF p_i = fl32(fl64(C_i) / fl64(T)) (ITS's division by
+ * the total, functions.cuh:171-174, over the inclusive CDF; uniform:
+ * p_i = fl32((i + 1) / d)), and a draw r in [0, 1) picks the first i with
+ * p_i > r, clamped to d - 1 (the reference's search could return d). */
+static inline float remap_p(const its_t *s, int64_t i) {
+  if (s->mode == 1) return (float)((double)s->C[i] / (double)s->T);
+  if (s->mode == 2) return (float)((double)s->Cf[i] / (double)s->Tf);
+  return (float)((double)(i + 1) / (double)s->d);
+}
+
+static inline int64_t remap_pick(const its_t *s, float r) {
+  int64_t lo = 0, hi = s->d;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (remap_p(s, mid) > r) hi = mid; else lo = mid + 1;
+  }
+  return lo < s->d ? lo : s->d - 1;
+}
+
 typedef struct {
   uint64_t *cdf;   /* maxdeg */
   uint32_t *stamp; /* maxdeg */
@@ -272,7 +293,9 @@ static void work_free(work_t *w) { free(w->cdf); free(w->stamp); }
  *    philox(key; (j<<16)|a, pos, sample, DOM_SELECT<<28 | request<<8 | hop)
  *    and is accepted unless already taken (bitmap_search, functions.cuh:45-70);
  *    after ORC_RESELECT attempts (the 400-try bound, functions.cuh:844) the
- *    first untaken index scanning upward cyclically from the last draw wins. */
+ *    first untaken index scanning upward cyclically from the last draw wins;
+ *  - dedup == 2: the same slots with heur_normalize's region remap on
+ *    collisions (above). */
 static int32_t select_core(const orc_graph *g, int32_t bias, int32_t dedup, int32_t F, int64_t v,
                            uint64_t key, uint32_t request, uint32_t hop, uint32_t sample, uint32_t pos,
                            int32_t *idx, work_t *w) {
@@ -294,6 +317,44 @@ static int32_t select_core(const orc_graph *g, int32_t bias, int32_t dedup, int3
     return F;
   }
   if (++w->cur == 0) { memset(w->stamp, 0, sizeof(uint32_t) * (size_t)d); w->cur = 1; }
+  if (dedup == 2) {
+    /* region remap: per slot, bounds lb = 0, hb = 1; each attempt draws r,
+     * picks; on a collision with the picked interval [A, B) it remaps r into
+     * the rest of [lb, hb) with heur_normalize's float sequence
+     * (functions.cuh:813-843, the reads of degree_l at selected - 1 and
+     * selected + 1 taken as A = p_{i-1} (0 for i = 0) and B = p_i) and
+     * picks again; after ORC_RESELECT attempts the cyclic fallback. */
+    for (int32_t j = 0; j < F; ++j) {
+      float lb = 0.0f, hb = 1.0f;
+      int64_t i = 0;
+      int found = 0;
+      for (uint32_t a = 0; a < ORC_RESELECT && !found; ++a) {
+        draw(key, ((uint32_t)j << 16) | a, pos, sample, c3, o);
+        float r = (float)(o[0] >> 8) * 5.9604644775390625e-08f;
+        i = remap_pick(&s, r);
+        if (w->stamp[i] != w->cur) { found = 1; break; }
+        const float A = i > 0 ? remap_p(&s, i - 1) : 0.0f;
+        const float B = remap_p(&s, i);
+        const float lambda = (A - lb) / (A - lb + hb - B);
+        const float delta = (B - A) / (hb - lb);
+        r = r / lambda;
+        if (r < A) {
+          hb = A;
+        } else {
+          r = r + delta;
+          lb = B;
+        }
+        i = remap_pick(&s, r);
+        if (w->stamp[i] != w->cur) found = 1;
+      }
+      if (!found) {
+        do { i = (i + 1 == d) ? 0 : i + 1; } while (w->stamp[i] == w->cur);
+      }
+      w->stamp[i] = w->cur;
+      idx[j] = (int32_t)i;
+    }
+    return F;
+  }
   for (int32_t j = 0; j < F; ++j) {
     int64_t i = 0;
     int found = 0;

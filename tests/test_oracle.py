@@ -257,6 +257,61 @@ def test_fallback_known_answers(O, w, F, want):
     assert (r["dst"].reshape(30, F) == np.array(want) + 1).all()
 
 
+# Region-remap mode (dedup = 2, heur_normalize).  Known answer, derived by
+# hand from the definition in IEEE arithmetic: FP32 weights [0,1,0,0] (p =
+# [0,1,1,1]), F = 3.  Slot 0 draws index 1.  Slot 1 collides at 1: A = 0,
+# B = 1, lambda = 0/0 = NaN, so r = NaN, NaN < A is false, r = NaN + 1 and
+# lb = 1; no p_i exceeds NaN, so the pick clamps to 3, which is free.  Slot
+# 2 (lb = 0, hb = 1 again) remaps to 3 (taken); from then on lb = hb = 1:
+# lambda = -1/-1 = 1, delta = 1/0 = inf, r = inf, pick 3 (taken), for all
+# 400 attempts; the cyclic fallback from 3 takes 0.
+def test_remap_known_answer(O):
+    off, adj = directed_star(4)
+    g = O.OracleGraph(off, adj, weights=np.array([0, 1, 0, 0], np.float32))
+    for s in range(10):
+        assert list(O.select_source(g, 0, 3, bias=O.BIAS_WEIGHT, dedup=2, sample=s, pos=s)) == [1, 3, 0]
+    # same law through the integer path (degree bias, one neighbour with edges)
+    off, adj = directed_star(4, {1: 5})
+    g = O.OracleGraph(off, adj)
+    r = O.sample(g, np.zeros(20, np.int32), [3], bias=O.BIAS_DEGREE, dedup=2)
+    assert (r["dst"].reshape(20, 3) == np.array([1, 3, 0]) + 1).all()
+
+
+@pytest.mark.parametrize("bias", [0, 1, 2])
+def test_remap_invariants_and_first_pick(O, bias):
+    off, adj = O.rmat(11, 8)
+    w = O.weights(len(adj), key=3)
+    g = O.OracleGraph(off, adj, weights=w)
+    seeds = O.seeds(len(off) - 1, 300)
+    r = O.sample(g, seeds, [6, 4], bias=bias, dedup=2)
+    check_sample_invariants(off, adj, seeds, [6, 4], r)
+    # slot 0 never collides: its pick is the first i with p_i > u, p the
+    # normalised CDF, computed here with numpy (searchsorted over float32 of
+    # float64 quotients)
+    deg = np.diff(off)
+    for s in range(50):
+        v = int(seeds[s])
+        d = int(deg[v])
+        if d <= 6:
+            continue
+        nb = adj[off[v]:off[v + 1]]
+        if bias == 0:
+            p = (np.arange(1, d + 1, dtype=np.float64) / d).astype(np.float32)
+        elif bias == 1:
+            C = np.cumsum(deg[nb].astype(np.uint64))
+            p = (C.astype(np.float64) / float(C[-1])).astype(np.float32) if C[-1] else None
+        else:
+            c = np.cumsum(w[off[v]:off[v + 1]], dtype=np.float32)
+            p = (c.astype(np.float64) / float(c[-1])).astype(np.float32)
+        if p is None:
+            continue
+        o = O.philox([0, 0, s, (4 << 28) | 0], [42, 0])
+        u = np.float32(int(o[0]) >> 8) * np.float32(2.0 ** -24)
+        i = min(int(np.searchsorted(p, u, side="right")), d - 1)
+        a = r["hop_off"][0, s]
+        assert r["dst"][a] == nb[i], (s, v)
+
+
 def test_fallback_path_distinct(O):
     # hub neighbour with 99.9% of the weight: later slots exhaust the 400
     # attempts and take the cyclic fallback; picks stay distinct, k = F
