@@ -285,8 +285,10 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, int64_t n) {
 // gs: group size for the hop's fanout (group_size_for; 0 = the whole-wave
 // path), used for the kinds that need no per-entry CDF build (uniform, FP32,
 // and integer biases when the CDF cache is on).  IDX: the graph view carries
-// the CDF cache and its 16-ary index (searched through the index).
-template <int KIND, bool IDX>
+// the CDF cache and its 16-ary index (searched through the index).  KEYS:
+// kKeys32 when no integer cache of the view is 64-bit (the 64-bit searches
+// compiled out of the group path).
+template <int KIND, bool IDX, int KEYS>
 __global__ __launch_bounds__(256) void k_expand_plain(GraphDev G, csaw_graph_view hv, const ReqDev* __restrict__ R,
                                                       const int32_t* __restrict__ gs2req, HopDev H, int gs) {
   constexpr bool kGroupOk = KIND != kInt || IDX;            // no per-entry CDF build needed
@@ -308,7 +310,7 @@ __global__ __launch_bounds__(256) void k_expand_plain(GraphDev G, csaw_graph_vie
     const int cnt = (int)min(bs, n - e0);
     if constexpr (kGroupOk) {
       if (grp) {
-        group_expand_batch<KIND>(G, hv, R, gs2req, H, nullptr, e0, cnt, gs, gl);
+        group_expand_batch<KIND, KEYS>(G, hv, R, gs2req, H, nullptr, e0, cnt, gs, gl);
         continue;
       }
     }
@@ -468,8 +470,14 @@ struct ChunkCursor {
 __device__ __forceinline__ ChunkCursor next_chunk(const ExpandArgs& A, int q, uint32_t* slot) {
   __syncthreads();
   if (threadIdx.x == 0) {
-    const uint32_t i = atomicAdd(&A.qhead[q], 1u);
-    *slot = i < A.qlen[q] ? A.queues[(int64_t)q * A.queue_cap + i] : 0xFFFFFFFFu;
+    uint32_t s = 0xFFFFFFFFu;
+    const uint32_t len = A.qlen[q];
+    // an exhausted queue is seen without an atomic on its head
+    if (__hip_atomic_load(&A.qhead[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < len) {
+      const uint32_t i = atomicAdd(&A.qhead[q], 1u);
+      if (i < len) s = A.queues[(int64_t)q * A.queue_cap + i];
+    }
+    *slot = s;
   }
   __syncthreads();
   ChunkCursor c;
@@ -478,17 +486,45 @@ __device__ __forceinline__ ChunkCursor next_chunk(const ExpandArgs& A, int q, ui
   return c;
 }
 
-// Wave-level work queue: lane 0 claims the next chunk of queue q; every lane
-// gets the same answer (no workgroup barrier).
-__device__ __forceinline__ ChunkCursor next_chunk_wave(const ExpandArgs& A, int q) {
-  uint32_t ci = 0xFFFFFFFFu;
-  if (lane_id() == 0) {
-    const uint32_t i = atomicAdd(&A.qhead[q], 1u);
-    if (i < A.qlen[q]) ci = A.queues[(int64_t)q * A.queue_cap + i];
-  }
+// Wave-level work queue.  Every claim is an atomic on the queue's one head
+// word, and atomics on one address serialise in its L2 channel, so a wave
+// claims a run of chunks at once while the queue is long: a quarter of its
+// share of what is left (at most kClaimMax), down to single chunks at the
+// tail.  A head already past the end is seen with a plain (L2) load and
+// costs no atomic.  Lane 0 claims; every lane gets the same answer (no
+// workgroup barrier).  Which wave expands which chunk never changes the
+// edges (their positions come from the prefix sums).
+constexpr uint32_t kClaimMax = 16;
+struct WaveClaim {
+  uint32_t next = 0, end = 0;
+};
+
+__device__ __forceinline__ ChunkCursor next_chunk_wave(const ExpandArgs& A, int q, WaveClaim& wc) {
   ChunkCursor c;
+  c.valid = false;
+  c.ci = 0xFFFFFFFFu;
+  if (wc.next >= wc.end) {
+    uint32_t b = 0, e = 0;
+    if (lane_id() == 0) {
+      const uint32_t len = A.qlen[q];
+      const uint32_t h = __hip_atomic_load(&A.qhead[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (h < len) {
+        const uint32_t nw = gridDim.x * (blockDim.x / kWave);
+        uint32_t k = (len - h) / (4u * nw);
+        k = k < 1u ? 1u : (k > kClaimMax ? kClaimMax : k);
+        b = atomicAdd(&A.qhead[q], k);
+        e = b < len ? min(b + k, len) : b;
+      }
+    }
+    wc.next = readlane_u(b, 0);
+    wc.end = readlane_u(e, 0);
+    if (wc.next >= wc.end) return c;
+  }
+  uint32_t ci = 0;
+  if (lane_id() == 0) ci = A.queues[(int64_t)q * A.queue_cap + wc.next];
   c.ci = readlane_u(ci, 0);
-  c.valid = c.ci != 0xFFFFFFFFu;
+  c.valid = true;
+  ++wc.next;
   return c;
 }
 
@@ -524,8 +560,9 @@ __global__ __launch_bounds__(256) void k_expand_thread(GraphDev G, csaw_graph_vi
   __shared__ uint32_t lds[kThreadIntCap * 256];
   const int q = CSAW_VAR_THREAD * 3 + KIND;
   // each wave claims its own chunks; lane k handles entries k, k+64, ...
+  WaveClaim wc;
   for (;;) {
-    const ChunkCursor cc = next_chunk_wave(A, q);
+    const ChunkCursor cc = next_chunk_wave(A, q, wc);
     if (!cc.valid) break;
     const uint64_t t0 = wall_clock64();
     const Chunk c = H.chunks[cc.ci];
@@ -558,8 +595,9 @@ __global__ __launch_bounds__(256) void k_expand_wave(GraphDev G, csaw_graph_view
   uint32_t* gl = nullptr;
   if (KIND == kF32 || (G32 && KIND == kInt)) gl = glds + wid * kGroupLdsWords;
   else if (KIND == kInt) gl = reinterpret_cast<uint32_t*>(cdf + wid * kWaveIntCap);
+  WaveClaim wc;
   for (;;) {
-    const ChunkCursor cc = next_chunk_wave(A, q);
+    const ChunkCursor cc = next_chunk_wave(A, q, wc);
     if (!cc.valid) break;
     const uint64_t t0 = wall_clock64();
     const Chunk c = H.chunks[cc.ci];
@@ -591,8 +629,9 @@ __global__ __launch_bounds__(256) void k_expand_remap(GraphDev G, csaw_graph_vie
   __shared__ uint64_t cdf[KIND == kInt ? 4 * kWaveIntCap : 1];
   __shared__ uint32_t bms[4 * (kWaveBitmapCap / 32)];
   const int wid = threadIdx.x / kWave;
+  WaveClaim wc;
   for (;;) {
-    const ChunkCursor cc = next_chunk_wave(A, kQueueRemap0 + KIND);
+    const ChunkCursor cc = next_chunk_wave(A, kQueueRemap0 + KIND, wc);
     if (!cc.valid) break;
     const uint64_t t0 = wall_clock64();
     const Chunk c = H.chunks[cc.ci];
@@ -689,8 +728,9 @@ __global__ __launch_bounds__(256) void k_expand_block(GraphDev G, csaw_graph_vie
 // group copying neighbour i (coalesced reads and writes)
 __global__ __launch_bounds__(256) void k_expand_copy(GraphDev G, csaw_graph_view hv, const ReqDev* __restrict__ R,
                                                      const int32_t* __restrict__ gs2req, HopDev H, ExpandArgs A) {
+  WaveClaim wc;
   for (;;) {
-    const ChunkCursor cc = next_chunk_wave(A, kQueueCopy);
+    const ChunkCursor cc = next_chunk_wave(A, kQueueCopy, wc);
     if (!cc.valid) break;
     const Chunk c = H.chunks[cc.ci];
     const int gs = group_size_for(c.max_f, H.group_min);
@@ -755,11 +795,14 @@ int launch_expand_plain(const LaunchCfg& L, int kind, const GraphDev& G, const c
   if (L.grid_override > 0) grid = L.grid_override;
   // the CDF cache (and its index) is in the view: plain sampling with cdf_cache
   const bool idx = G.cdf_idx || G.int_idx32[0] || G.int_idx32[1] || G.int_idx[0] || G.int_idx[1];
-  if (kind == kUniform) k_expand_plain<kUniform, false><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, gs);
-  else if (kind == kInt && idx) k_expand_plain<kInt, true><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, gs);
-  else if (kind == kInt) k_expand_plain<kInt, false><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, 0);
-  else if (idx) k_expand_plain<kF32, true><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, gs);
-  else k_expand_plain<kF32, false><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, gs);
+  const bool wide = G.int_cdf[0] || G.int_cdf[1];  // a 64-bit integer cache in the view
+  if (kind == kUniform) k_expand_plain<kUniform, false, kKeys32><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, gs);
+  else if (kind == kInt && idx && !wide)
+    k_expand_plain<kInt, true, kKeys32><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, gs);
+  else if (kind == kInt && idx) k_expand_plain<kInt, true, kKeysAny><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, gs);
+  else if (kind == kInt) k_expand_plain<kInt, false, kKeysAny><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, 0);
+  else if (idx) k_expand_plain<kF32, true, kKeys32><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, gs);
+  else k_expand_plain<kF32, false, kKeys32><<<grid, 256, 0, L.stream>>>(G, hv, R, gs2req, H, gs);
   CSAW_LAUNCH_CHECK();
   return CSAW_OK;
 }
