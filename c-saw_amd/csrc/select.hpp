@@ -745,23 +745,57 @@ __device__ void group_expand_batch(const GraphDev& G, const csaw_graph_view& hv,
                  : *cdf_index_first_block<uint32_t>(C, user ? G.int_idx32[1] : G.int_idx32[0], G.E, mine.o, mine.d);
     }
   }
+  // Entries that copy all their d <= F neighbours take no group round: the
+  // wave writes their edges in a flat pass, lane s handling copied edge s of
+  // the batch (its entry found by a binary search over the running edge
+  // counts), and the selecting entries are compacted so that every group
+  // round expands one.
+  const bool live = lane < count && mine.d > 0 && mine.F > 0;
+  const bool cp = live && (mine.flags & 1u) && mine.d <= mine.F;
+  const uint32_t kc = cp ? (uint32_t)mine.d : 0u;
+  const uint32_t kci = wave_incl_scan_u32(kc);
+  const uint32_t ncopy = readlane_u(kci, kWave - 1);
+  for (uint32_t s0 = 0; s0 < ncopy; s0 += kWave) {
+    const uint32_t s = s0 + (uint32_t)lane;
+    int L = 0;  // first lane whose running count exceeds s
+#pragma unroll
+    for (int b = 32; b >= 1; b >>= 1)
+      if (shfl_abs_u(kci, L + b - 1) <= s) L += b;
+    L = L < kWave ? L : kWave - 1;
+    const uint32_t before = shfl_abs_u(kci, L) - shfl_abs_u(kc, L);
+    const int32_t cv = shfl_abs(mine.v, L), cg = shfl_abs(mine.g, L);
+    const int64_t co = (int64_t)shfl_abs_u64((uint64_t)mine.o, L);
+    const int64_t cpos = (int64_t)shfl_abs_u64((uint64_t)mine.pos, L);
+    if (s < ncopy) {
+      const int64_t i = (int64_t)(s - before);
+      H.src[cpos + i] = cv;
+      H.dst[cpos + i] = Update(&hv, G.adj[co + i], cv);
+      H.smp[cpos + i] = cg;
+    }
+  }
+  const bool selects = live && !cp;
+  const uint64_t sm = ballot(selects);
+  const int nsel = __popcll(sm);
+  // lane r receives the lane of the r-th selecting entry (ds_permute pushes
+  // every lane to a distinct destination: selecting lanes to their rank,
+  // the others after them)
+  const int rank = __popcll(sm & ((1ull << lane) - 1ull));
+  const int dest = selects ? rank : nsel + (lane - rank);
+  const int sel_lane = __builtin_amdgcn_ds_permute(dest << 2, lane);
   // Software pipeline: a round's neighbour load is consumed (written out) only
   // after the next round's selection, so its latency overlaps that round's
   // searches instead of adding to them.
   bool have = false;
   int32_t pv = 0, pg = 0, pnb = 0;
   int64_t pp = 0;
-  for (int e0 = 0; e0 < count; e0 += NG) {
-    const int e = gi < NG ? e0 + gi : count;
-    const GEntry en = shfl_gentry(mine, e < count ? e : 0);
+  for (int e0 = 0; e0 < nsel; e0 += NG) {
+    const int e = gi < NG ? e0 + gi : nsel;
+    const int src = shfl_abs(sel_lane, e < nsel ? e : 0);
+    const GEntry en = shfl_gentry(mine, src);
     int32_t idx = -1;
-    if (e < count && en.d > 0 && en.F > 0) {
-      if ((en.flags & 1u) && en.d <= en.F) {
-        idx = gl < en.d ? gl : -1;
-      } else {
-        const int32_t res = group_select<KIND, KEYS>(G, en, gs, gl, gbase, gmask, lc, cap);
-        idx = gl < en.F ? res : -1;
-      }
+    if (e < nsel) {
+      const int32_t res = group_select<KIND, KEYS>(G, en, gs, gl, gbase, gmask, lc, cap);
+      idx = gl < en.F ? res : -1;
     }
     if (have) {
       H.src[pp] = pv;
