@@ -164,8 +164,8 @@ def bench_single(a, C, O, name, scale, ef, weights, seeds_n, fanouts, bias, vari
         off, adj, *w = g.csr(weights=weights)
         og = O.OracleGraph(off, adj, weights=w[0] if w else None)
         ce, cs, ok, chk = oracle_check_sample(O, og, specs, plain.result, a.cpu_sample if seeds_n > 10000 else 1.0,
-                                              min(16, os.cpu_count() or 1))
-        cpu = {"value": ce / cs, "unit": "edges/s", "cores": min(16, os.cpu_count() or 1), "kind": "port",
+                                              host_cpus()["threads_used"])
+        cpu = {"value": ce / cs, "unit": "edges/s", "cores": host_cpus()["threads_used"], "kind": "port",
                "sample": f"{ce} edges ({cs:.1f} s); oracle == GPU in {ok}/{chk} hops"}
     line(f"SEPS ({name})", e / tp, "edges/s", a.steps, a.warmup, tp * 1e3,
          {"workload": name, "seeds": seeds_n, "fanouts": fanouts, "bias": bias, "edges_per_step": e},
@@ -195,9 +195,9 @@ def bench_walk(a, C, O):
         og = O.OracleGraph(off, adj, weights=w)
         m = int(len(seeds) * a.cpu_sample)
         t0 = time.perf_counter()
-        Wo, so = O.walk(og, seeds[:m], 80, bias=O.BIAS_WEIGHT, nthreads=min(16, os.cpu_count() or 1))
+        Wo, so = O.walk(og, seeds[:m], 80, bias=O.BIAS_WEIGHT, nthreads=host_cpus()["threads_used"])
         cs = time.perf_counter() - t0
-        cpu = {"value": so / cs, "unit": "steps/s", "cores": min(16, os.cpu_count() or 1), "kind": "port",
+        cpu = {"value": so / cs, "unit": "steps/s", "cores": host_cpus()["threads_used"], "kind": "port",
                "sample": f"first {m} walkers ({so} steps, {cs:.1f} s); equal to the GPU rows: "
                          f"{bool(np.array_equal(Wo, W[:m]))}"}
     line("walk steps/s (C3)", steps / (kms * 1e-3), "steps/s", a.steps, a.warmup, kms,
@@ -277,11 +277,11 @@ def bench_stream(a, C, O, CM):
         og = O.OracleGraph(off, adj)
         m = 256
         t1 = time.perf_counter()
-        r = O.sample(og, seeds[:m], [15, 10, 5], bias=O.BIAS_DEGREE, nthreads=min(16, os.cpu_count() or 1))
+        r = O.sample(og, seeds[:m], [15, 10, 5], bias=O.BIAS_DEGREE, nthreads=host_cpus()["threads_used"])
         cs = time.perf_counter() - t1
         okh = all(np.array_equal(res["dst"][res["hop_off"][h, 0]:res["hop_off"][h, m]],
                                  r["dst"][r["hop_off"][h, 0]:r["hop_off"][h, m]]) for h in range(3))
-        cpu = {"value": len(r["src"]) / cs, "unit": "edges/s", "cores": min(16, os.cpu_count() or 1),
+        cpu = {"value": len(r["src"]) / cs, "unit": "edges/s", "cores": host_cpus()["threads_used"],
                "kind": "port", "sample": f"first {m} seeds ({len(r['src'])} edges, {cs:.1f} s); equal to the "
                                          f"streamed GPU edges: {okh}"}
     line("SEPS streaming (C5)", st.edges / (dms * 1e-3), "edges/s", a.steps, a.warmup, dms,
@@ -436,6 +436,12 @@ def main():
     p, _, _ = measure("batched_heuristic", mode=C.MODE_BATCHED, sched=C.SCHED_HEURISTIC, use_graph=True,
                       tuning=tuning, cdf_cache=True, serial_expand=a.serial_expand)
     p.close()
+    # the best fixed policy: one variant forced on every chunk (the wave
+    # variant with the bitmap taken set is the fastest single variant on C4)
+    p, _, _ = measure("batched_forced_wave_bitmap", mode=C.MODE_BATCHED, sched=C.SCHED_FORCED,
+                      variant=C.VAR_WAVE_BITMAP, use_graph=True, tuning=tuning, cdf_cache=True,
+                      serial_expand=a.serial_expand)
+    p.close()
     batched, t_bat, same = measure("batched_learned", mode=C.MODE_BATCHED, sched=C.SCHED_MLP, mlp=mlp,
                                    use_graph=True, tuning=tuning, cdf_cache=True, serial_expand=a.serial_expand)
     log(f"plain {t_plain * 1e3:.1f} ms  batched {t_bat * 1e3:.1f} ms  edges {edges}  digests equal {same}/256")
@@ -502,7 +508,10 @@ def main():
                           "batching (plain+cache -> batched+heuristic)":
                               runs["plain_cdf_cache"]["ms_per_step"] / runs["batched_heuristic"]["ms_per_step"],
                           "learned schedule (batched+heuristic -> batched+learned)":
-                              runs["batched_heuristic"]["ms_per_step"] / runs["batched_learned"]["ms_per_step"]},
+                              runs["batched_heuristic"]["ms_per_step"] / runs["batched_learned"]["ms_per_step"],
+                          "learned schedule vs the best fixed variant (batched, wave bitmap forced)":
+                              runs["batched_forced_wave_bitmap"]["ms_per_step"]
+                              / runs["batched_learned"]["ms_per_step"]},
         "bit_exact": {"requests_equal": int(same), "requests": len(specs)},
         "one_time_ms": {"bias_data (FP32 CDF, hub CDFs)": prep_ms,
                         "cdf_cache (integer CDF cache, search indexes, vertex records)": cache_ms},
