@@ -70,6 +70,16 @@ def get_tuning(C, g, specs, mlp, retrain):
     return t
 
 
+def uniform_tuning(C):
+    """Launch shapes of the plain-uniform instance (a swept grid: 32
+    expansion and 16 count workgroups per CU, 1.57 ms against 1.65 ms at the
+    defaults of 8 and 8)."""
+    t = C.Tuning()
+    for k, v in json.load(open(os.path.join(MODELS, "tuning_gfx950_uniform.json"))).items():
+        setattr(t, k, int(v))
+    return t
+
+
 def run_timed(C, plan, steps, warmup):
     for _ in range(warmup):
         plan.run(sync=True)
@@ -148,17 +158,23 @@ def oracle_check_sample(O, og, specs, gpu_results, frac, nthreads):
 
 
 def bench_single(a, C, O, name, scale, ef, weights, seeds_n, fanouts, bias, variant=None):
-    """C1 / C2 / plain-uniform instance: one request, plain and batched executors."""
+    """C1 / C2 / plain-uniform instance: one request, plain and batched
+    executors (batched: the learned schedule with the shipped cost model, or
+    the variant given)."""
     g = C.Graph.rmat(scale, ef, weights=weights)
     specs = [{"seeds": g.seeds(seeds_n), "fanouts": fanouts, "bias": bias, "request": 0}]
-    plain = C.Plan(g, specs, mode=C.MODE_PLAIN)
-    sched = C.SCHED_FORCED if variant is not None else C.SCHED_HEURISTIC
-    bat = C.Plan(g, specs, mode=C.MODE_BATCHED, sched=sched, variant=variant or 0, use_graph=True, cdf_cache=True)
+    ut = uniform_tuning(C) if (bias == C.BIAS_UNIFORM and seeds_n > 100000) else None
+    plain = C.Plan(g, specs, mode=C.MODE_PLAIN, tuning=ut)
+    import cost_model as CM
+    mlp, _ = CM.load(C, CM.DEFAULT_MODEL)
+    sched = C.SCHED_FORCED if variant is not None else C.SCHED_MLP
+    bat = C.Plan(g, specs, mode=C.MODE_BATCHED, sched=sched, variant=variant or 0, use_graph=True, cdf_cache=True,
+                 mlp=mlp)
     tp = run_timed(C, plain, a.steps, a.warmup)
     tb = run_timed(C, bat, a.steps, a.warmup)
     e = plain.n_edges(0)
     assert plain.digest(0) == bat.digest(0)
-    roof, kt = roofline_of(C.Plan(g, specs, mode=C.MODE_PLAIN, timing=True))
+    roof, kt = roofline_of(C.Plan(g, specs, mode=C.MODE_PLAIN, timing=True, tuning=ut))
     cpu = None
     if not a.no_cpu:
         off, adj, *w = g.csr(weights=weights)
@@ -370,7 +386,7 @@ def main():
         O = ge.load_oracle() if not a.no_cpu else None
         if a.workload == "c1":
             bench_single(a, C, O, "C1: RMAT-16 ef8, 1000 samples, FS 1, NS 2, depth 2, uniform, bitmap",
-                         16, 8, False, 1000, [2, 2], C.BIAS_UNIFORM, variant=C.VAR_WAVE_BITMAP)
+                         16, 8, False, 1000, [2, 2], C.BIAS_UNIFORM)
         elif a.workload == "c2":
             bench_single(a, C, O, "C2: RMAT-20 ef5, 8192 seeds, {25,10}, degree bias", 20, 5, False, 8192, [25, 10],
                          C.BIAS_DEGREE)
@@ -469,11 +485,12 @@ def main():
     # the plain-uniform roofline instance on the same graph: 1,048,576 seeds,
     # fanouts {25,10}, uniform bias, plain sampler
     uspec = [{"seeds": g.seeds(1 << 20), "fanouts": [25, 10], "bias": C.BIAS_UNIFORM, "request": 0}]
-    up = C.Plan(g, uspec, mode=C.MODE_PLAIN)
+    ut = uniform_tuning(C)
+    up = C.Plan(g, uspec, mode=C.MODE_PLAIN, tuning=ut)
     t_u = run_timed(C, up, a.steps, a.warmup)
     e_u = up.n_edges(0)
     up.close()
-    uroof, ukt = roofline_of(C.Plan(g, uspec, mode=C.MODE_PLAIN, timing=True))
+    uroof, ukt = roofline_of(C.Plan(g, uspec, mode=C.MODE_PLAIN, timing=True, tuning=ut))
     ceiling = gather_ceiling()
     log(f"uniform instance {t_u * 1e3:.3f} ms ({e_u} edges), {uroof['kernel']} {uroof['frac']:.3f} of HBM peak; "
         f"random-read ceiling {ceiling['reads_per_s']:.3g}/s")

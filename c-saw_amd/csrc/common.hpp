@@ -133,7 +133,13 @@ struct HopDev {
   int32_t expand_waves; // waves of one expand launch (sizes chunks; 0: whole buckets)
   int32_t chunk_min;    // smallest chunk piece (a power of two, kChunkMin .. kBlockEntries)
   int32_t group_min;    // smallest sub-wave group (kGroupMin .. 32)
+  int32_t only_kind;    // 1 + a bias kind: expand only that kind's entries (0: all)
 };
+
+// Hops of a batched plan with at most this many frontier entries skip the
+// binning and the variant kernels (whose fixed costs dominate a small hop):
+// one whole-hop kernel per bias kind expands them in frontier order.
+constexpr int64_t kSmallHopEntries = 65536;
 
 // Entries per chunk for a hop of n frontier entries: whole 1024-entry buckets
 // when the hop gives every expand wave several of them, smaller pieces (down

@@ -238,6 +238,7 @@ static HopDev make_hop(csaw_plan* p, Unit& u, int h) {
   H.expand_waves = p->L.num_cu * p->L.expand_per_cu * 4;  // 4 waves per 256-lane workgroup
   H.chunk_min = p->L.chunk_min;
   H.group_min = p->L.group_min;
+  H.only_kind = 0;
   return H;
 }
 
@@ -246,9 +247,16 @@ static int run_unit(csaw_plan* p, Unit& u, hipStream_t st, TimingHook* th) {
   const csaw_graph_view hv = p->g->hook_view();
   LaunchCfg L = p->L;
   L.stream = st;
-  const bool bin = p->opts.mode == CSAW_MODE_BATCHED;
+  const bool batched = p->opts.mode == CSAW_MODE_BATCHED;
   for (int h = 0; h < u.depth_max; ++h) {
     HopDev H = make_hop(p, u, h);
+    // a small hop of a batched plan (its entry bound known on the host) is
+    // not binned: one whole-hop kernel per bias kind, as in the plain sampler
+    // (schedules that ask for particular variants, forced or random, and
+    // chunk logging keep the binned path)
+    const bool small = batched && u.nmax[h] <= kSmallHopEntries && !p->opts.log_chunks &&
+                       (p->opts.sched == CSAW_SCHED_HEURISTIC || p->opts.sched == CSAW_SCHED_MLP);
+    const bool bin = batched && !small;
     if (bin) {
       if (th) th->begin(kKernFill, st);
       CSAW_TRY(launch_fill(L, p->counters, 0u, kCtrCount - 1));  // keeps the flags word
@@ -310,18 +318,30 @@ static int run_unit(csaw_plan* p, Unit& u, hipStream_t st, TimingHook* th) {
         p->pending_log_hops.push_back(h);
       }
     } else {
-      int kind = 0;
-      for (int k = 0; k < 3; ++k)
-        if (u.kinds[k]) kind = k;
-      // sub-wave group size from the largest fanout of the hop
-      int fmax = 0;
-      for (int ri : u.reqs)
-        if (p->reqs[ri].depth > h) fmax = std::max(fmax, p->reqs[ri].fanouts[h]);
-      // region-remap requests resolve their slots with whole waves (gs = 0)
-      const int gs = u.remap ? 0 : group_size_for(fmax, L.group_min);
-      if (th) th->begin(kKernPlain, st);
-      CSAW_TRY(launch_expand_plain(L, kind, G, hv, u.d_req, u.d_gs2req, H, u.nmax[h], gs));
-      if (th) th->end(kKernPlain, st);
+      // one launch per bias kind present (one for a plain unit, whose single
+      // request has one kind); with several kinds each launch expands only
+      // its own kind's entries
+      int nk = 0;
+      for (int k = 0; k < 3; ++k) nk += u.kinds[k] ? 1 : 0;
+      for (int kind = 0; kind < 3; ++kind) {
+        if (!u.kinds[kind]) continue;
+        // sub-wave group size from the largest fanout of the kind's requests
+        int fmax = 0;
+        bool remap = false;
+        for (int ri : u.reqs) {
+          const csaw_request& r = p->reqs[ri];
+          if (r.depth > h && bias_kind(r.bias) == kind) {
+            fmax = std::max(fmax, r.fanouts[h]);
+            remap |= r.dedup == 2;
+          }
+        }
+        // region-remap requests resolve their slots with whole waves (gs = 0)
+        const int gs = remap ? 0 : group_size_for(fmax, L.group_min);
+        H.only_kind = nk > 1 ? 1 + kind : 0;
+        if (th) th->begin(kKernPlain, st);
+        CSAW_TRY(launch_expand_plain(L, kind, G, hv, u.d_req, u.d_gs2req, H, u.nmax[h], gs));
+        if (th) th->end(kKernPlain, st);
+      }
     }
     if (bin && p->opts.log_chunks) {
       // chunk log must be read before the next hop overwrites chunk storage

@@ -457,6 +457,7 @@ __device__ void wave_expand_batch(const GraphDev& G, const csaw_graph_view& hv, 
   mine.d = 0;
   if (lane < count) {
     mine = load_entry(G, R, gs2req, H, order ? (int64_t)order[first + lane] : first + lane);
+    if (H.only_kind && bias_kind(mine.bias) + 1 != H.only_kind) mine.d = 0;  // another kind's launch
     if (mine.d > 0 && mine.F > 0 && !(mine.dedup && mine.d <= mine.F))
       build = picker_global(G, mine, KIND, kWaveIntCap, pm) ? 0 : 1;
   }
@@ -729,7 +730,12 @@ __device__ void group_expand_batch(const GraphDev& G, const csaw_graph_view& hv,
   mine.F = 0;
   uint32_t warm = 0;
   if (lane < count) {
-    mine = make_gentry<KIND>(G, load_entry(G, R, gs2req, H, order ? (int64_t)order[first + lane] : first + lane));
+    const Entry en0 = load_entry(G, R, gs2req, H, order ? (int64_t)order[first + lane] : first + lane);
+    mine = make_gentry<KIND>(G, en0);
+    if (H.only_kind && bias_kind(en0.bias) + 1 != H.only_kind) {  // another kind's launch
+      mine.d = 0;
+      mine.flags &= ~0xF0u;  // no search state, no warm-up load
+    }
     // Every entry's first search block (or, for small degrees, its CDF) is
     // loaded now, for all 64 entries at once, so the rounds below find it in
     // cache instead of each waiting for memory.

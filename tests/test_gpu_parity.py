@@ -191,6 +191,30 @@ def test_batched_equals_plain_and_oracle(C, O, g14):
             same(bat.result(i), want, f"batched req {i}")
 
 
+def test_small_hops_unbinned(C, O, g14):
+    # batched plans run hops of at most 65,536 frontier entries without
+    # binning (one whole-hop kernel per bias kind, each skipping the other
+    # kinds' entries): mixed kinds, the user hooks, region remap, with and
+    # without the CDF cache, heuristic and learned schedules, graph replays
+    import os, sys
+    sys.path.insert(0, os.path.join(os.path.dirname(HERE), "c-saw_amd"))
+    import cost_model as CM
+    mlp, _ = CM.load(C, CM.DEFAULT_MODEL)
+    g, og, *_ = g14
+    specs = [{"seeds": g.seeds(300, request=i), "fanouts": [8, 4], "bias": i % 4, "dedup": 2 if i == 5 else 1,
+              "request": i} for i in range(8)]
+    want = [O.sample(og, s["seeds"], s["fanouts"], bias=s["bias"] if s["bias"] != 3 else O.BIAS_USER,
+                     dedup=s["dedup"], request=i) for i, s in enumerate(specs)]
+    for cache in (False, True):
+        for sched in (C.SCHED_HEURISTIC, C.SCHED_MLP):
+            p = C.Plan(g, specs, mode=C.MODE_BATCHED, cdf_cache=cache, use_graph=True, sched=sched, mlp=mlp)
+            for rep in range(2):
+                p.run()
+                for i in range(len(specs)):
+                    same(p.result(i), want[i], f"cache {cache} sched {sched} replay {rep} req {i}")
+            p.close()
+
+
 @pytest.mark.parametrize("cdf_cache,serial", [(False, False), (True, False), (True, True)])
 def test_hipgraph_replays(C, O, g14, cdf_cache, serial):
     # the captured hops fork each hop's variant kernels onto side streams
