@@ -620,7 +620,7 @@ __device__ __forceinline__ int32_t gpick_small(const GEntry& en, const uint32_t*
 // the group is lanes gbase .. gbase + gs - 1 (gmask)
 template <int KIND, int KEYS>
 __device__ int32_t group_select(const GraphDev& G, const GEntry& en, int gs, int gl, int gbase, uint64_t gmask,
-                                uint32_t* lc, int cap) {
+                                uint32_t* lc, int cap, int32_t* cl) {
   const int F = en.F;
   const int32_t d = en.d;
   const bool act = gl < F;
@@ -642,13 +642,20 @@ __device__ int32_t group_select(const GraphDev& G, const GEntry& en, int gs, int
   int32_t cand = pick(gl, 0, act);
   if (!act) cand = -1;
   if (!(en.flags & 1u)) return cand;
-  // duplicate check: shuffles issued eight at a time (one wait per eight)
+  // duplicate check: every lane writes its candidate into the group's
+  // 16-byte-aligned LDS slot, then reads the group's candidates back four
+  // at a time (one broadcast ds_read_b128 per four slots instead of one
+  // ds_bpermute per slot) and compares with the slots before its own
+  cl[gl] = cand;
+  wave_fence();
   bool dup = false;
-#pragma unroll 8
-  for (int i = 0; i + 1 < F; ++i) {
-    const int32_t ci = shfl_abs(cand, gbase + i);
-    dup |= (gl > i) & act & (cand == ci);
+  for (int i0 = 0; i0 + 1 < F; i0 += 4) {
+    const int4 q = *reinterpret_cast<const int4*>(cl + i0);
+    dup |= (gl > i0 && cand == q.x) | (gl > i0 + 1 && cand == q.y) | (gl > i0 + 2 && cand == q.z) |
+           (gl > i0 + 3 && cand == q.w);
   }
+  dup &= act;
+  wave_fence();  // every lane has read the slot before the next round writes it
   const uint64_t dm = ballot(dup) & gmask;
   if (dm == 0ull) return cand;
   const int j0 = __builtin_ctzll(dm) - gbase;
@@ -712,6 +719,10 @@ __device__ int32_t group_select(const GraphDev& G, const GEntry& en, int gs, int
 // cache.  lds: the wave's kGroupLdsWords words for small-degree CDFs (null:
 // searches only).
 constexpr int kGroupLdsWords = (kWave / kGroupMin) * kGroupSmallCap;
+// per-wave candidate slots: NG groups of gs lanes rounded up to four words,
+// NG * round4(gs) <= 64 + 3 * NG <= 112 for gs >= 4 (the idle remainder
+// lanes never select, so their slot pointer is never dereferenced)
+constexpr int kCandSlotWords = 128;
 
 template <int KIND, int KEYS = kKeysAny>
 __device__ void group_expand_batch(const GraphDev& G, const csaw_graph_view& hv, const ReqDev* __restrict__ R,
@@ -724,6 +735,12 @@ __device__ void group_expand_batch(const GraphDev& G, const csaw_graph_view& hv,
   const int gl = lane - gi * gs, gbase = gi * gs;
   const int cap = kGroupLdsWords / NG;  // LDS slot words per group
   uint32_t* lc = lds && gi < NG ? lds + gi * cap : nullptr;
+  // candidate slots of the duplicate check: per wave, each group's gs words
+  // rounded up to four (16-byte aligned); the idle remainder lanes' slot
+  // follows the groups'
+  __shared__ __align__(16) int32_t cand_slots[4][kCandSlotWords];
+  const int gstride = (gs + 3) & ~3;
+  int32_t* gcl = &cand_slots[(threadIdx.x / kWave) & 3][gi * gstride];
   const uint64_t gmask = gs >= kWave ? ~0ull : (((1ull << gs) - 1ull) << gbase);
   GEntry mine;
   mine.d = 0;
@@ -800,7 +817,7 @@ __device__ void group_expand_batch(const GraphDev& G, const csaw_graph_view& hv,
     const GEntry en = shfl_gentry(mine, src);
     int32_t idx = -1;
     if (e < nsel) {
-      const int32_t res = group_select<KIND, KEYS>(G, en, gs, gl, gbase, gmask, lc, cap);
+      const int32_t res = group_select<KIND, KEYS>(G, en, gs, gl, gbase, gmask, lc, cap, gcl);
       idx = gl < en.F ? res : -1;
     }
     if (have) {
