@@ -60,6 +60,15 @@ __host__ __device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) {
 #endif
 }
 
+// floor(a * b / 2^64) for a 32-bit b, exact with two 32x32->64 products:
+// a * b = (a_hi * b) * 2^32 + a_lo * b, and a_hi * b + floor(a_lo * b / 2^32)
+// < 2^64, so the floor of the whole divides out in two steps.
+__host__ __device__ __forceinline__ uint64_t mulhi64_32(uint64_t a, uint32_t b) {
+  const uint64_t lo = (uint64_t)(uint32_t)a * b;
+  const uint64_t hi = (a >> 32) * b;
+  return (hi + (lo >> 32)) >> 32;
+}
+
 // Uniform float in [0,1) with 24 random bits: (o0 >> 8) * 2^-24, exact.
 __host__ __device__ __forceinline__ float philox_u24(const U4& o) {
   return (float)(o.x >> 8) * 5.9604644775390625e-08f;

@@ -100,7 +100,7 @@ struct Picker {
   template <bool IDX>
   __device__ __forceinline__ int32_t pick(int64_t E, const U4& r) const {
     const uint64_t t = philox_t64(r);
-    if (mode == kPickUniform) return (int32_t)mulhi64(t, (uint64_t)d);
+    if (mode == kPickUniform) return (int32_t)mulhi64_32(t, (uint32_t)d);
     if (mode == kPickF32 || mode == kPickF32Idx) {
       const float x = __fmul_rn(philox_u24(r), Tf);
       const float* Cf = static_cast<const float*>(C);
@@ -546,17 +546,17 @@ template <int KIND, int KEYS = kKeysAny>
 __device__ __forceinline__ int32_t gpick(const GraphDev& G, const GEntry& en, const U4& r) {
   const uint32_t mode = (en.flags >> 4) & 15u;
   const uint64_t t = philox_t64(r);
-  if (KIND == kUniform || mode == kPickUniform) return (int32_t)mulhi64(t, (uint64_t)en.d);
+  if (KIND == kUniform || mode == kPickUniform) return (int32_t)mulhi64_32(t, (uint32_t)en.d);
   if constexpr (KIND == kF32) {
     const float x = __fmul_rn(philox_u24(r), __uint_as_float((uint32_t)en.T));
     if (mode == kPickF32Idx) return cdf_index_search<float>(G.cdf, G.cdf_idx, G.E, en.o, en.d, x);
     return (int32_t)upper_bound_dev<float>(G.cdf + en.o, en.d, x);
   }
   const bool user = (en.flags >> 1) & 1u;
-  const uint64_t x = mulhi64(t, en.T);
-  if (KEYS == kKeys32 || mode == kPickInt32Idx)
+  if (KEYS == kKeys32 || mode == kPickInt32Idx)  // uint32 cache: T < 2^32
     return cdf_index_search<uint32_t>(user ? G.int_cdf32[1] : G.int_cdf32[0], user ? G.int_idx32[1] : G.int_idx32[0],
-                                      G.E, en.o, en.d, (uint32_t)x);
+                                      G.E, en.o, en.d, (uint32_t)mulhi64_32(t, (uint32_t)en.T));
+  const uint64_t x = mulhi64(t, en.T);
   const uint64_t* C = user ? G.int_cdf[1] : G.int_cdf[0];
   if (mode == kPickIntIdx)
     return cdf_index_search<uint64_t>(C, user ? G.int_idx[1] : G.int_idx[0], G.E, en.o, en.d, x);
@@ -610,7 +610,7 @@ __device__ __forceinline__ int32_t gpick_small(const GEntry& en, const uint32_t*
     const float x = __fmul_rn(philox_u24(r), __uint_as_float((uint32_t)en.T));
     return (int32_t)upper_bound_dev<float>(reinterpret_cast<const float*>(lc), en.d, x);
   } else {
-    const uint32_t x = (uint32_t)mulhi64(philox_t64(r), en.T);
+    const uint32_t x = (uint32_t)mulhi64_32(philox_t64(r), (uint32_t)en.T);  // uint32 cache: T < 2^32
     return (int32_t)upper_bound_dev<uint32_t>(lc, en.d, x);
   }
 }
@@ -908,7 +908,7 @@ __device__ void expand_entry_thread(const GraphDev& G, const csaw_graph_view& hv
   }
   auto pick = [&](const U4& o) -> int32_t {
     const uint64_t t = philox_t64(o);
-    if (mode == kPickUniform) return (int32_t)mulhi64(t, (uint64_t)d);
+    if (mode == kPickUniform) return (int32_t)mulhi64_32(t, (uint32_t)d);
     if (mode == kPickLdsCdf) {
       // upper_bound over the lane's LDS column
       const float xf = KIND == kF32 ? __fmul_rn(philox_u24(o), Tf) : 0.f;
