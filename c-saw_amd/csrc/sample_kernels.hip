@@ -831,25 +831,25 @@ int launch_expand_chunks(const LaunchCfg& L, const GraphDev& G, const csaw_graph
   // integer caches of the view all uint32: the group-only builds
   const bool g32 = A.int_cache && !G.int_cdf[0] && !G.int_cdf[1];
   // WAVE kernels (most of the work) first, so that with several streams
-  // they start side by side
-  for (int kind = 0; kind < 3; ++kind) {
-    if (!kinds_present[kind]) continue;
-    if (kind == kUniform) {
-      CSAW_TRY(go(kKernWave, k_expand_wave<kUniform, true, true>, L.expand_per_cu));
-      CSAW_TRY(go(kKernWave, k_expand_wave<kUniform, false, true>, L.expand_per_cu));
-    } else if (kind == kInt) {
-      if (g32) {
-        CSAW_TRY(go(kKernWave, k_expand_wave<kInt, true, true>, L.expand_per_cu));
-        CSAW_TRY(go(kKernWave, k_expand_wave<kInt, false, true>, L.expand_per_cu));
-      } else {
-        CSAW_TRY(go(kKernWave, k_expand_wave<kInt, true, false>, L.expand_per_cu));
-        CSAW_TRY(go(kKernWave, k_expand_wave<kInt, false, false>, L.expand_per_cu));
-      }
-    } else {
-      CSAW_TRY(go(kKernWave, k_expand_wave<kF32, true, true>, L.expand_per_cu));
-      CSAW_TRY(go(kKernWave, k_expand_wave<kF32, false, true>, L.expand_per_cu));
-    }
+  // they start side by side.  Within them, the variant that carries most of
+  // each kind's work starts first, one per stream: the taken-set bitmap for
+  // the biased kinds (their high-degree, collision-heavy entries), the list
+  // for uniform entries.  Their second variants queue behind them on the
+  // four hardware queues instead of delaying a heavy kernel's start (on 256
+  // mixed requests over RMAT-22 the FP32 bitmap kernel otherwise started
+  // 1.2-1.8 ms into hops 2 and 3).
+  if (kinds_present[kInt]) {
+    if (g32) CSAW_TRY(go(kKernWave, k_expand_wave<kInt, true, true>, L.expand_per_cu));
+    else CSAW_TRY(go(kKernWave, k_expand_wave<kInt, true, false>, L.expand_per_cu));
   }
+  if (kinds_present[kF32]) CSAW_TRY(go(kKernWave, k_expand_wave<kF32, true, true>, L.expand_per_cu));
+  if (kinds_present[kUniform]) CSAW_TRY(go(kKernWave, k_expand_wave<kUniform, false, true>, L.expand_per_cu));
+  if (kinds_present[kInt]) {
+    if (g32) CSAW_TRY(go(kKernWave, k_expand_wave<kInt, false, true>, L.expand_per_cu));
+    else CSAW_TRY(go(kKernWave, k_expand_wave<kInt, false, false>, L.expand_per_cu));
+  }
+  if (kinds_present[kF32]) CSAW_TRY(go(kKernWave, k_expand_wave<kF32, false, true>, L.expand_per_cu));
+  if (kinds_present[kUniform]) CSAW_TRY(go(kKernWave, k_expand_wave<kUniform, true, true>, L.expand_per_cu));
   for (int kind = 0; kind < 3; ++kind) {
     if (!kinds_present[kind]) continue;
     if (kind == kUniform) {
